@@ -1,0 +1,184 @@
+/* CPU oracle, part 1: mechanism evaluation and the chemistry right-hand side.
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).  Restates, from the tables the
+ * mechanism compiler extracts from the reference mechanism text:
+ *   PRODRATES         combustion/HYDROGENF.f90:4-277 (grimech30F.f90:4-2580,
+ *                     DME39F.f90:4-1943)
+ *   GETLINDRATECOEFF  combustion/HYDROGENF.f90:280-316
+ *   COMPTHERMODATA    combustion/HYDROGENF.f90:514-673
+ *   finitechem_Cp / finitechem_compute_rhs  combustion/finitechem.f90:79-88, 93-175
+ * Expression order follows the Fortran statements so that rate coefficients
+ * are reproduced to the last bit where libm's exp/log10/pow agree; the
+ * production-rate sums are regrouped per reaction (net stoichiometry), which
+ * changes only rounding.
+ */
+#include <math.h>
+#include <string.h>
+
+#include "oracle_int.h"
+#include "../tiger_nga_amd/csrc/mech_h2.h"
+#include "../tiger_nga_amd/csrc/mech_gri30.h"
+#include "../tiger_nga_amd/csrc/mech_dme39.h"
+
+/* unsuffixed Fortran literals are binary32 values widened to double */
+#define F4(x) ((double)(x##f))
+
+const tgr_mech_tables *orc_tables(int mech) {
+  switch (mech) {
+    case ORC_H2: return &h2_tables;
+    case ORC_GRI30: return &gri30_tables;
+    case ORC_DME39: return &dme39_tables;
+    default: return 0;
+  }
+}
+
+int orc_mech_info(int mech, orc_info *info) {
+  const tgr_mech_tables *t = orc_tables(mech);
+  if (!t) return -1;
+  info->nsp = t->nsp;
+  info->nrxn = t->nrxn;
+  info->nslot = t->nslot;
+  info->n_third = t->n_third;
+  return 0;
+}
+
+int orc_molar_mass(int mech, double *W) {
+  const tgr_mech_tables *t = orc_tables(mech);
+  if (!t) return -1;
+  memcpy(W, t->W, sizeof(double) * t->nsp);
+  return 0;
+}
+
+const char *orc_species_name(int mech, int k) {
+  const tgr_mech_tables *t = orc_tables(mech);
+  if (!t || k < 0 || k >= t->nsp) return 0;
+  return t->species[k];
+}
+
+/* GETLINDRATECOEFF (HYDROGENF.f90:280-316), ITROE = 1 branch. */
+static double lindrate(double T, double P, double k0, double kinf, double fc, double concin) {
+  const double R = F4(8314.34);
+  double conc = (concin > 0.0) ? concin : P / (R * T);
+  double ntmp = 0.75 - F4(1.27) * log10(fc);
+  double ccoeff = -F4(0.4) - F4(0.67) * log10(fc);
+  double dcoeff = F4(0.14);
+  double pr = k0 * conc / fmax(kinf, 1.0e-60);
+  double lgk = log10(pr);
+  double f = (lgk + ccoeff) / (ntmp - dcoeff * (lgk + ccoeff));
+  f = pow(fc, 1.0 / (f * f + 1.0));
+  return kinf * f * pr / (1.0 + pr);
+}
+
+static double arrh(const double *a, double lt, double rt) {
+  /* K = A * exp(b*LT - E/RT), written with e = -E */
+  return a[0] * exp(a[1] * lt + a[2] / rt);
+}
+
+static double troe_fc(const tgr_rxn *r, double T) {
+  double fc = 0.0;
+  for (int i = 0; i < r->nfc; ++i) {
+    double x = r->fc_kind[i] == 0 ? -(T / r->fc_d[i]) : -(r->fc_d[i] / T);
+    fc = fc + r->fc_c[i] * exp(x);
+  }
+  return fc;
+}
+
+int orc_prodrates_t(const tgr_mech_tables *t, const double *C, double T, double P, double *cdot,
+                    double *K, double *W, double *Mout) {
+  double M[64];
+  const int nsp = t->nsp;
+  const double lt = log(T);
+  const double rt = F4(8314.34) * T;
+  for (int j = 0; j < t->n_third; ++j) {
+    double s = 0.0;
+    for (int k = 0; k < nsp; ++k) {
+      double e = t->eff[j * nsp + k];
+      if (e != 0.0) s += e * C[k];
+    }
+    M[j] = s;
+    if (Mout) Mout[j] = s;
+  }
+  for (int k = 0; k < nsp; ++k) cdot[k] = 0.0;
+  for (int i = 0; i < t->nrxn; ++i) {
+    const tgr_rxn *r = &t->rxn[i];
+    double kf, kb = 0.0;
+    if (r->kind == 2) {
+      double fc = troe_fc(r, T);
+      double conc = r->m >= 0 ? M[r->m] : -1.0;
+      kf = lindrate(T, P, arrh(r->f0, lt, rt), arrh(r->f, lt, rt), fc, conc);
+      if (r->rev) kb = lindrate(T, P, arrh(r->b0, lt, rt), arrh(r->b, lt, rt), fc, conc);
+    } else {
+      kf = arrh(r->f, lt, rt);
+      if (r->rev) kb = arrh(r->b, lt, rt);
+    }
+    double wf = kf, wb = kb;
+    for (int q = 0; q < r->nr; ++q) wf = wf * C[r->reac[q]];
+    for (int q = 0; q < r->np; ++q) wb = wb * C[r->prod[q]];
+    if (r->kind == 1) {
+      wf = wf * M[r->m];
+      wb = wb * M[r->m];
+    }
+    if (K) {
+      K[r->slot_f] = kf;
+      if (r->rev) K[r->slot_b] = kb;
+    }
+    if (W) {
+      W[r->slot_f] = wf;
+      if (r->rev) W[r->slot_b] = wb;
+    }
+    double q = r->rev ? wf - wb : wf;
+    for (int n = 0; n < r->nnu; ++n) {
+      const tgr_nu *u = &t->nu[r->nu0 + n];
+      cdot[u->sp] += u->nu * q;
+    }
+  }
+  return 0;
+}
+
+int orc_prodrates(int mech, const double *C, double T, double P, double *cdot, double *K, double *W,
+                  double *M) {
+  const tgr_mech_tables *t = orc_tables(mech);
+  if (!t) return -1;
+  return orc_prodrates_t(t, C, T, P, cdot, K, W, M);
+}
+
+void orc_thermo_t(const tgr_mech_tables *t, double T, double *h, double *cp) {
+  const int nsp = t->nsp;
+  const double *base = t->thermo + (T > t->tmid ? 0 : nsp * 13);
+  for (int k = 0; k < nsp; ++k) {
+    const double *c = base + 13 * k;
+    h[k] = c[0] * (T * (c[1] + T * (c[2] + T * (c[3] + T * (c[4] + T * c[5])))) + c[6]);
+    cp[k] = c[7] * (c[8] + T * (c[9] + T * (c[10] + T * (c[11] + T * c[12]))));
+  }
+}
+
+int orc_thermo(int mech, double T, double *h, double *cp) {
+  const tgr_mech_tables *t = orc_tables(mech);
+  if (!t) return -1;
+  orc_thermo_t(t, T, h, cp);
+  return 0;
+}
+
+/* finitechem_compute_rhs, low-Mach constant-pressure branch (finitechem.f90:117-170) */
+void orc_rhs_t(const tgr_mech_tables *t, double p_gp, const double *sol, double *rhs) {
+  double h[64], cp[64], C[64], cdot[64];
+  const int nsp = t->nsp;
+  double rho = 0.0;
+  for (int k = 0; k < nsp; ++k) rho += sol[k];
+  const double T = sol[nsp] / rho;
+  orc_thermo_t(t, T, h, cp);
+  double cpm = 0.0;
+  for (int k = 0; k < nsp; ++k) cpm += (sol[k] / rho) * cp[k];
+  for (int k = 0; k < nsp; ++k) C[k] = sol[k] / t->W[k];
+  orc_prodrates_t(t, C, T, p_gp, cdot, 0, 0, 0);
+  for (int k = 0; k < nsp; ++k) rhs[k] = cdot[k] * t->W[k];
+  double tmp = 0.0;
+  for (int k = 0; k < nsp; ++k) tmp = tmp - h[k] * rhs[k];
+  rhs[nsp] = tmp / cpm;
+}
+
+int orc_rhs(int mech, double p_gp, const double *sol, double *rhs) {
+  const tgr_mech_tables *t = orc_tables(mech);
+  if (!t) return -1;
+  orc_rhs_t(t, p_gp, sol, rhs);
+  return 0;
+}

@@ -1,0 +1,87 @@
+/* CPU oracle for the finite-rate chemistry step -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C FP64 restatement of the reference path, used by tests/, by
+ * __graft_entry__.smoke() and by bench.py's CPU-baseline leg as the checker
+ * and as the timed CPU stand-in for the reference.  Nothing in the product
+ * (tiger_nga_amd/) links or calls it.
+ *
+ * Parts and the reference they restate (paths relative to the reference root):
+ *   orc_prodrates   combustion/HYDROGENF.f90:4-277 (PRODRATES), 280-316
+ *                   (GETLINDRATECOEFF); same routines in grimech30F.f90 /
+ *                   DME39F.f90.  Pinned by golden vectors of the reference
+ *                   mechanism files compiled with flang (tests/golden).
+ *   orc_thermo      combustion/HYDROGENF.f90:514-673 (COMPTHERMODATA). Pinned
+ *                   likewise, plus dh/dT = cp.
+ *   orc_rhs         combustion/finitechem.f90:93-175 (finitechem_compute_rhs),
+ *                   79-88 (finitechem_Cp).  Pinned by composition with the
+ *                   golden production rates and by invariants (sum of mass
+ *                   rates = 0 up to binary32 molar masses, enthalpy balance).
+ *   orc_integrate   the SUNDIALS CVODE 2.5.0 BDF/Newton/dense solver behind
+ *                   FCVODE (finitechem.f90:1014-1034), restated from the
+ *                   published CVODE algorithm (Hindmarsh et al., ACM TOMS
+ *                   31(3), 2005; CVODE 2.5 user guide).  SUNDIALS is not in the
+ *                   reference tree: pinned against scipy's Radau/BDF at tight
+ *                   tolerance, closed forms (dt -> 0, frozen cold cells) and
+ *                   invariants, not against CVODE's own step sequence.
+ *   orc_source_chemistry  finitechem.f90:962-1071 (pack, integrate, unpack,
+ *                   heat release and H2O/CO/CO2 source diagnostics).
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_H2 = 0, ORC_GRI30 = 1, ORC_DME39 = 2 };
+
+typedef struct {
+  int nsp, nrxn, nslot, n_third;
+} orc_info;
+
+typedef struct {
+  double rtol, atol;  /* scalar tolerances (finitechem.f90:266-267 defaults 1e-6, 1e-12) */
+  int max_steps;      /* MAX_NSTEPS (finitechem.f90:1025: 10000) */
+  int max_order;      /* BDF maximum order (CVODE default 5) */
+} orc_opts;
+
+/* per-cell integrator counters (same meaning as CVODE's) */
+typedef struct {
+  int64_t nst, nfe, nfe_dq, nje, nsetups, nni, netf, ncfn, nlu;
+  int64_t status; /* 0 ok, <0 failure code (see oracle/cvode_bdf.c) */
+} orc_stats;
+
+int orc_mech_info(int mech, orc_info *info);
+int orc_molar_mass(int mech, double *W);
+const char *orc_species_name(int mech, int k);
+
+/* PRODRATES: C[nsp] kmol/m3, T K, P Pa -> cdot[nsp]; optional per-slot K, W
+ * (reference slot order) and third-body sums M. */
+int orc_prodrates(int mech, const double *C, double T, double P, double *cdot,
+                  double *K, double *W, double *M);
+/* COMPTHERMODATA: h[nsp] J/kg, cp[nsp] J/kg/K */
+int orc_thermo(int mech, double T, double *h, double *cp);
+/* finitechem_compute_rhs: sol = [rho*Y (nsp), rho*T], p_gp = 0.5(P_th+P_th_old) */
+int orc_rhs(int mech, double p_gp, const double *sol, double *rhs);
+/* one cell, t: 0 -> dt, sol in/out (neq = nsp+1) */
+int orc_integrate(int mech, const orc_opts *o, double dt, double p_gp, double *sol, orc_stats *st);
+/* batch of packed cells: sol[cell*neq + i]; threads >= 1 */
+int orc_integrate_batch(int mech, const orc_opts *o, double dt, double p_gp, int64_t ncell,
+                        double *sol, orc_stats *per_cell, int nthreads);
+
+/* Fortran-layout grid: arrays RHO_(imino_:imaxo_, jmino_:jmaxo_, kmino_:kmaxo_),
+ * SC_(..., nscalar), mask(imino:imaxo, jmino:jmaxo); isc_* are 1-based. */
+typedef struct {
+  int imin_, imax_, jmin_, jmax_, kmin_, kmax_;
+  int imino_, imaxo_, jmino_, jmaxo_, kmino_, kmaxo_;
+  int imino, imaxo, jmino, jmaxo;
+  int nscalar, isc_sc, isc_T;
+} orc_grid;
+
+int orc_source_chemistry(int mech, const orc_opts *o, double dt, double p_thermo, double p_thermo_old,
+                         const orc_grid *g, double *RHO_, double *SC_, const int *mask,
+                         double *HR_gp, double *chemsrc_gp, orc_stats *total, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
