@@ -1,0 +1,119 @@
+/* tiger_chem: MI355X-native finite-rate chemistry step (C ABI).
+ *
+ * Drop-in for the chemistry step of NGA / Tiger_NGA: the body of
+ * finitechem_source_chemistry (reference combustion/finitechem.f90:962-1071),
+ * the SUNDIALS CVODE integration it drives (finitechem.f90:987-1068) and the
+ * link-time mechanism plugin (combustion/mechanism.f90:17-108, one of
+ * HYDROGENF.f90 / grimech30F.f90 / DME39F.f90 selected in
+ * combustion/Makefile:3).  Plain pointers and sizes only; a Fortran host binds
+ * these with ISO_C_BINDING (tiger_nga_amd/fortran/tiger_chem_mod.f90).
+ *
+ * Conventions: every function returns 0 on success and a negative TGR_E_*
+ * code on failure (tgr_chem_last_error() gives the message).  Arrays use the
+ * Fortran layout of the reference: RHO_(imino_:imaxo_, jmino_:jmaxo_,
+ * kmino_:kmaxo_), SC_(..., nscalar) with i fastest, mask(imino:imaxo,
+ * jmino:jmaxo); scalar indices are 1-based as in the reference.
+ */
+#ifndef TIGER_CHEM_H
+#define TIGER_CHEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TGR_E_ARG (-1)      /* invalid argument */
+#define TGR_E_HIP (-2)      /* HIP runtime error */
+#define TGR_E_MECH (-3)     /* unknown or unsupported mechanism */
+#define TGR_E_NOMEM (-4)    /* device allocation failed */
+
+typedef struct tgr_chem tgr_chem;
+
+/* Extents of the calling rank's block (reference geometry/partition.f90:53-90,
+ * geometry/masks.f90:55) and the scalar slots of the species and T
+ * (finitechem.f90:19 isc_sc, combustion isc_T). */
+typedef struct {
+  int imin_, imax_, jmin_, jmax_, kmin_, kmax_;       /* interior cells, inclusive */
+  int imino_, imaxo_, jmino_, jmaxo_, kmino_, kmaxo_; /* extents of RHO_ / SC_ */
+  int imino, imaxo, jmino, jmaxo;                     /* extents of mask */
+  int nscalar, isc_sc, isc_T;                         /* 1-based */
+} tgr_grid;
+
+/* Per-call integrator statistics, summed over the cells of the call (the
+ * reference discards CVODE's iout counters and return code). */
+typedef struct {
+  int64_t cells;                  /* cells integrated (interior, mask /= 1) */
+  int64_t nst, nfe, nfe_dq, nje;  /* steps, RHS evals, DQ-Jacobian RHS evals, Jacobians */
+  int64_t nsetups, nni, netf, ncfn, nlu; /* setups, Newton iters, error/conv failures, LU factorisations */
+  int64_t failed;                 /* cells whose integration stopped early (state = last step) */
+  double kernel_ms;               /* device time of the step kernel */
+} tgr_stats;
+
+/* Mechanism sizes (reference getters GETNSPECIES/GETNSPECS/GETNREACTIONS,
+ * HYDROGENF.f90:408-450) plus the table sizes the kernels use. */
+typedef struct {
+  int nsp, nrxn, nslot, n_third, n_falloff, n_third_body_rxn, n_reversible, n_nu;
+  int n_fc_terms; /* sum of Fc exponential terms over fall-off reactions */
+} tgr_mech_info;
+
+/* Create a context: mechanism tables on the current HIP device, solver
+ * options, work buffers.  Replaces finitechem_init (finitechem.f90:184-319:
+ * GETNSPECIES, GETMOLARMASS, rtol/atol parser keys, defaults 1e-6 / 1e-12)
+ * and the CVODE set-up of finitechem.f90:1014-1028 (FCVMALLOC, STOP_TIME,
+ * MAX_NSTEPS = 10000, FCVDENSE).  mech: "h2" | "gri30" | "dme39". */
+int tgr_chem_create(const char *mech, double rtol, double atol, int max_steps, tgr_chem **out);
+
+/* Release the context (FCVFREE, finitechem.f90:1068). */
+int tgr_chem_destroy(tgr_chem *ctx);
+
+/* GETNSPECIES / GETNREACTIONS and table sizes. */
+int tgr_chem_mech_info(const tgr_chem *ctx, tgr_mech_info *info);
+
+/* GETSPECIESNAMES (HYDROGENF.f90:389-405): name of species k (0-based). */
+int tgr_chem_species_name(const tgr_chem *ctx, int k, char *buf, int len);
+
+/* GETMOLARMASS (HYDROGENF.f90:366-386): W[nsp] in kg/kmol. */
+int tgr_chem_molar_mass(const tgr_chem *ctx, double *W);
+
+/* finitechem_source_chemistry(RHO_, SC_) (finitechem.f90:962-1071) on HOST
+ * arrays: advances species and T of every interior cell with mask /= 1 over
+ * dt at P = (p_thermo + p_thermo_old)/2, in place, and writes HR_gp and
+ * chemsrc_gp(..., 1:3) = d(rho Y)/dt of H2O, CO, CO2 (0 where the mechanism
+ * lacks the species).  mask, HR_gp, chemsrc_gp may be NULL.  Synchronous.
+ * stats may be NULL. */
+int tgr_chem_source_chemistry(tgr_chem *ctx, double dt, double p_thermo, double p_thermo_old,
+                              const tgr_grid *g, double *RHO_, double *SC_, const int *mask,
+                              double *HR_gp, double *chemsrc_gp, tgr_stats *stats);
+
+/* Same on DEVICE arrays (HBM-resident fields), enqueued on `stream` (a
+ * hipStream_t, NULL = default).  With stats != NULL the call waits for the
+ * step and fills them; with NULL it returns after enqueueing. */
+int tgr_chem_source_chemistry_device(tgr_chem *ctx, double dt, double p_thermo, double p_thermo_old,
+                                     const tgr_grid *g, double *RHO_, double *SC_, const int *mask,
+                                     double *HR_gp, double *chemsrc_gp, void *stream, tgr_stats *stats);
+
+/* PRODRATES (HYDROGENF.f90:4-277) for n states, device arrays, SoA:
+ * C[nsp][n] kmol/m3, T[n] K, P[n] Pa -> cdot[nsp][n] kmol/m3/s. */
+int tgr_chem_prodrates_device(tgr_chem *ctx, int64_t n, const double *C, const double *T, const double *P,
+                              double *cdot, void *stream);
+
+/* finitechem_compute_rhs (finitechem.f90:93-175) for n states, device arrays:
+ * sol[neq][n] = [rho*Y, rho*T] -> rhs[neq][n] at pressure p_gp. */
+int tgr_chem_rhs_device(tgr_chem *ctx, int64_t n, double p_gp, const double *sol, double *rhs, void *stream);
+
+/* Tuning knobs (no counterpart in the reference):
+ *   "persistent"  1: persistent grid pulling cells from a device counter (default), 0: one block per cell
+ *   "grid"        persistent grid size in blocks (0: occupancy x CUs)
+ *   "max_order"   BDF maximum order 1..5 (CVODE default 5)
+ *   "max_steps"   MAX_NSTEPS */
+int tgr_chem_set_option(tgr_chem *ctx, const char *key, int64_t value);
+
+/* Message of the last failing call on this thread. */
+const char *tgr_chem_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TIGER_CHEM_H */

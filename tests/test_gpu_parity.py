@@ -1,0 +1,138 @@
+"""GPU parity of the HIP chemistry path against the reference golden vectors
+and the CPU oracle.  All calls go through the C ABI (libtiger_chem.so)."""
+import os
+
+import numpy as np
+import pytest
+
+from tiger_nga_amd import inputs
+from tiger_nga_amd.mech import meta
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+MECHS = ("h2", "gri30", "dme39")
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    assert torch.cuda.is_available(), "GPU test without a HIP device"
+    return torch
+
+
+@pytest.fixture(scope="module")
+def chem():
+    from tiger_nga_amd.finitechem import FiniteChem
+    cache = {}
+
+    def get(mech, rtol=1e-6, atol=1e-12):
+        key = (mech, rtol, atol)
+        if key not in cache:
+            cache[key] = FiniteChem(mech, rtol, atol)
+        return cache[key]
+    return get
+
+
+@pytest.mark.parametrize("mech", MECHS)
+def test_prodrates_vs_reference_golden(mech, chem, torch_dev):
+    """PRODRATES of the compiled reference mechanism (flang) vs the HIP kernel;
+    tolerance relative to sum |W| over all rate slots (net rates cancel)."""
+    torch = torch_dev
+    d = np.load(os.path.join(GOLD, "%s_mech.npz" % mech))
+    fc = chem(mech)
+    C = torch.tensor(np.ascontiguousarray(d["C"].T), device="cuda")
+    T = torch.tensor(d["T"], device="cuda")
+    P = torch.tensor(d["P"], device="cuda")
+    cd = fc.prodrates(C, T, P).cpu().numpy().T
+    scale = np.abs(d["W"]).sum(axis=1, keepdims=True)
+    err = np.abs(cd - d["CDOT"]) / np.maximum(scale, 1e-300)
+    print("%s prodrates max err / sum|W| = %.3e" % (mech, err.max()))
+    assert err.max() < 1e-12
+
+
+@pytest.mark.parametrize("mech,workload", [("h2", "h2"), ("gri30", "gri30"), ("dme39", "dme39")])
+def test_rhs_vs_oracle(mech, workload, chem, oracle, torch_dev):
+    torch = torch_dev
+    _, Y, T, rho, P, _ = inputs.states(workload, n=256, seed=7)
+    sol = inputs.packed_state(Y, T, rho)
+    fc = chem(mech)
+    out = fc.compute_rhs(torch.tensor(np.ascontiguousarray(sol.T), device="cuda"), P).cpu().numpy().T
+    W = meta.molar_mass(mech)
+    worst = 0.0
+    for c in range(sol.shape[0]):
+        ref = oracle.rhs(mech, sol[c], P)
+        Cc = sol[c, :-1] / W
+        _, _, Wsl, _ = oracle.prodrates(mech, Cc, sol[c, -1] / sol[c, :-1].sum(), P)
+        scale_sp = np.abs(Wsl).sum() * W.max()
+        h, cp = oracle.thermo(mech, sol[c, -1] / sol[c, :-1].sum(), len(W))
+        cpm = (sol[c, :-1] / sol[c, :-1].sum() * cp).sum()
+        scale_T = scale_sp * np.abs(h).max() * len(W) / cpm
+        e_sp = np.abs(out[c, :-1] - ref[:-1]).max() / scale_sp
+        e_T = abs(out[c, -1] - ref[-1]) / scale_T
+        worst = max(worst, e_sp, e_T)
+    print("%s rhs max scaled err %.3e" % (mech, worst))
+    assert worst < 1e-12
+
+
+def _grid_case(mech, workload, nx, ny, nz, nover, seed, with_mask):
+    from tiger_nga_amd.finitechem import Grid
+    _, Y, T, rho, P, dt = inputs.states(workload, n=nx * ny * nz, seed=seed)
+    RHO_, SC_, isc_sc, isc_T = inputs.fortran_fields(Y, T, rho, nx, ny, nz, nover=nover, extra_scalars=1)
+    # put an unrelated scalar first so that isc_sc > 1
+    SC_ = np.asfortranarray(np.concatenate([SC_[..., -1:], SC_[..., :-1]], axis=3))
+    g = Grid.block(nx, ny, nz, SC_.shape[3], isc_sc + 1, isc_T + 1, nover=nover)
+    mask = None
+    if with_mask:
+        mask = np.zeros(g.mask_shape, dtype=np.int32, order="F")
+        mask[nover + 1, :] = 1  # a wall row
+    return g, RHO_, SC_, mask, P, dt
+
+
+def _rel_err(a, b, floor):
+    return np.abs(a - b) / np.maximum(np.abs(b), floor)
+
+
+@pytest.mark.parametrize("mech,workload,rtol,atol,ytol,ttol", [
+    ("h2", "h2", 1e-6, 1e-12, None, None),
+    ("h2", "h2", 1e-10, 1e-16, 1e-8, 1e-6),
+    ("gri30", "gri30", 1e-6, 1e-12, None, None),
+    ("gri30", "gri30", 1e-10, 1e-16, 1e-8, 1e-6),
+])
+def test_step_vs_oracle(mech, workload, rtol, atol, ytol, ttol, chem, oracle, torch_dev):
+    """finitechem_source_chemistry on a ghosted, masked block: HIP vs oracle.
+
+    Tight tolerance (parity mode): mass fractions to rel 1e-8 (floor 1e-12)
+    and T to 1e-6 K.  Reference tolerance: both run the same CVODE-semantics
+    BDF, so most cells agree far below rtol; bounded by a few rtol overall."""
+    nx, ny, nz = 4, 4, 4
+    g, RHO_, SC_, mask, P, dt = _grid_case(mech, workload, nx, ny, nz, 2, 11, True)
+    fc = chem(mech, rtol, atol)
+    RHO_g, SC_g = RHO_.copy(order="F"), SC_.copy(order="F")
+    HR_g = np.full(RHO_.shape, 3.0, order="F")
+    CS_g = np.full(RHO_.shape + (3,), 1e-60, order="F")
+    st = fc.source_chemistry(RHO_g, SC_g, g, dt, P, P, mask=mask, HR_gp=HR_g, chemsrc_gp=CS_g)
+    SC_o = SC_.copy(order="F")
+    HR_o = np.full(RHO_.shape, 3.0, order="F")
+    CS_o = np.full(RHO_.shape + (3,), 1e-60, order="F")
+    sto = oracle.source_chemistry(mech, RHO_, SC_o, g, dt, P, P, mask=mask, HR_gp=HR_o, chemsrc_gp=CS_o,
+                                  rtol=rtol, atol=atol)
+    nsp = len(meta.species(mech))
+    sp = slice(g.isc_sc - 1, g.isc_sc - 1 + nsp)
+    eY = _rel_err(SC_g[..., sp], SC_o[..., sp], 1e-12)
+    eT = np.abs(SC_g[..., g.isc_T - 1] - SC_o[..., g.isc_T - 1])
+    print("\n%s rtol=%g: cells %d, steps gpu %d cpu %d, nfe gpu %d cpu %d, failed %d/%d"
+          % (mech, rtol, st["cells"], st["nst"], sto["nst"], st["nfe"], sto["nfe"], st["failed"], sto["status"]))
+    print("   max rel Y err %.3e  max |dT| %.3e K  HR rel %.3e" % (
+        eY.max(), eT.max(), (np.abs(HR_g - HR_o) / np.maximum(np.abs(HR_o), 1.0)).max()))
+    # untouched: ghosts, wall cells, the unrelated scalar
+    assert np.array_equal(SC_g[..., 0], SC_[..., 0])
+    assert np.array_equal(SC_g[0], SC_[0]) and np.array_equal(SC_g[:, :, -1], SC_[:, :, -1])
+    wall = mask.astype(bool)
+    assert np.array_equal(SC_g[wall], SC_[wall]) and np.all(HR_g[wall] == 3.0)
+    assert st["failed"] == 0 and sto["status"] == 0
+    assert st["cells"] == nx * ny * nz - ny * nz
+    if ytol is not None:
+        assert eY.max() < ytol and eT.max() < ttol
+    else:
+        assert eY.max() < 50 * rtol and eT.max() < 1e-3

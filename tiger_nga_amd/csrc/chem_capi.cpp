@@ -1,0 +1,504 @@
+// C-ABI layer of the chemistry step: context, device tables, launches.
+// Entry points are declared in include/tiger_chem.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tiger_chem.h"
+#include "chem_kernels.h"
+#include "mech_dme39.h"
+#include "mech_gri30.h"
+#include "mech_h2.h"
+
+using tgr::DevMech;
+using tgr::StepParams;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(x)                                                                 \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess) return fail(TGR_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+const tgr_mech_tables* find_mech(const char* name) {
+  if (!name) return nullptr;
+  if (!strcmp(name, "h2")) return &h2_tables;
+  if (!strcmp(name, "gri30")) return &gri30_tables;
+  if (!strcmp(name, "dme39")) return &dme39_tables;
+  return nullptr;
+}
+
+// Device buffer that grows on demand.
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes) {
+    if (bytes <= cap) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipMalloc(&p, bytes) != hipSuccess) return fail(TGR_E_NOMEM, "hipMalloc failed");
+    cap = bytes;
+    return 0;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+}  // namespace
+
+struct tgr_chem {
+  const tgr_mech_tables* t = nullptr;
+  int nsp = 0, neq = 0;
+  double rtol = 1e-6, atol = 1e-12;
+  int max_steps = 10000, qmax = 5;
+  int persistent = 1;
+  int64_t grid_override = 0;
+  int device = 0;
+  int n_cu = 0;
+  tgr_mech_info info{};
+  DevMech dm{};
+  void* d_tables = nullptr;
+  unsigned int* d_queue = nullptr;
+  unsigned long long* d_totals = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // staging for the host-array entry point
+  DevBuf b_rho, b_sc, b_mask, b_hr, b_chem;
+};
+
+namespace {
+
+// Builds the device tables: reactions re-ordered fall-off first (so only the
+// first 64-reaction round of a wave diverges), SoA arrays, species padded to
+// 64 lanes, and the CDOT gather schedule.
+int build_tables(tgr_chem* c) {
+  const tgr_mech_tables* t = c->t;
+  const int nsp = t->nsp, nr = t->nrxn;
+  if (nsp + 1 > tgr::kMaxNeq || nr > 320 || t->n_third > 64)
+    return fail(TGR_E_MECH, "mechanism exceeds the wave-per-cell limits");
+  std::vector<int> order;
+  for (int r = 0; r < nr; ++r)
+    if (t->rxn[r].kind == 2) order.push_back(r);
+  const int nfo = (int)order.size();
+  for (int r = 0; r < nr; ++r)
+    if (t->rxn[r].kind != 2) order.push_back(r);
+  if (nfo > 255) return fail(TGR_E_MECH, "too many fall-off reactions");
+
+  std::vector<double> dbl;
+  std::vector<int> ints;
+  auto dalloc = [&](size_t n) { size_t o = dbl.size(); dbl.resize(o + n, 0.0); return o; };
+  auto ialloc = [&](size_t n) { size_t o = ints.size(); ints.resize(o + n, 0); return o; };
+
+  const size_t oW = dalloc(nsp), oiW = dalloc(nsp), oTh = dalloc(2 * 13 * 64), oEff = dalloc((size_t)nsp * 64);
+  const size_t oArr = dalloc(6 * (size_t)nr), oFo = dalloc(15 * (size_t)std::max(nfo, 1));
+  const size_t oMeta = ialloc(nr), oSp = ialloc(6 * (size_t)nr), oFoN = ialloc(std::max(nfo, 1));
+  const size_t oGs = ialloc(64), oGc = ialloc(64), oSf = ialloc(64), oSn = ialloc(64);
+
+  for (int k = 0; k < nsp; ++k) {
+    dbl[oW + k] = t->W[k];
+    dbl[oiW + k] = 1.0 / t->W[k];
+  }
+  for (int rg = 0; rg < 2; ++rg)
+    for (int k = 0; k < nsp; ++k)
+      for (int q = 0; q < 13; ++q) dbl[oTh + (size_t)(rg * 13 + q) * 64 + k] = t->thermo[(size_t)(rg * nsp + k) * 13 + q];
+  for (int j = 0; j < t->n_third; ++j)
+    for (int k = 0; k < nsp; ++k) dbl[oEff + (size_t)k * 64 + j] = t->eff[(size_t)j * nsp + k];
+
+  std::vector<std::vector<std::pair<int, double>>> bysp(nsp);
+  for (int g = 0; g < nr; ++g) {
+    const tgr_rxn& x = t->rxn[order[g]];
+    for (int q = 0; q < 3; ++q) {
+      dbl[oArr + (size_t)q * nr + g] = x.f[q];
+      dbl[oArr + (size_t)(3 + q) * nr + g] = x.b[q];
+    }
+    const int fo = (x.kind == 2) ? g : 0;
+    if (x.m + 1 > 127) return fail(TGR_E_MECH, "third-body index out of range");
+    ints[oMeta + g] = (x.kind & 3) | ((x.rev & 1) << 2) | ((x.nr & 3) << 3) | ((x.np & 3) << 5) |
+                      (((x.m + 1) & 127) << 7) | ((fo & 255) << 14);
+    for (int q = 0; q < 3; ++q) {
+      ints[oSp + (size_t)q * nr + g] = x.reac[q] < 0 ? 0 : x.reac[q];
+      ints[oSp + (size_t)(3 + q) * nr + g] = x.prod[q] < 0 ? 0 : x.prod[q];
+    }
+    if (x.kind == 2) {
+      const int nf = std::max(nfo, 1);
+      double* fo_ = &dbl[oFo];
+      for (int q = 0; q < 3; ++q) {
+        fo_[(size_t)q * nf + g] = x.f0[q];
+        fo_[(size_t)(3 + q) * nf + g] = x.b0[q];
+        fo_[(size_t)(6 + q) * nf + g] = x.fc_c[q];
+        fo_[(size_t)(9 + q) * nf + g] = x.fc_d[q];
+        fo_[(size_t)(12 + q) * nf + g] = (double)x.fc_kind[q];
+      }
+      ints[oFoN + g] = x.nfc;
+    }
+    for (int u = 0; u < x.nnu; ++u) {
+      const tgr_nu& e = t->nu[x.nu0 + u];
+      bysp[e.sp].push_back({g, e.nu});
+    }
+  }
+  // gather schedule: smallest chunk S with sum_k ceil(n_k/S) <= 64
+  int S = 1;
+  for (;; ++S) {
+    int lanes = 0;
+    for (int k = 0; k < nsp; ++k) lanes += ((int)bysp[k].size() + S - 1) / S;
+    if (lanes <= 64) break;
+  }
+  std::vector<int> gr;
+  std::vector<double> gnu;
+  int lane = 0, smax = 0;
+  for (int k = 0; k < nsp; ++k) {
+    const int nk = (int)bysp[k].size();
+    const int chunks = (nk + S - 1) / S;
+    ints[oSf + k] = lane;
+    ints[oSn + k] = chunks;
+    smax = std::max(smax, chunks);
+    for (int ch = 0; ch < chunks; ++ch, ++lane) {
+      ints[oGs + lane] = (int)gr.size();
+      const int lo = ch * S, hi = std::min(nk, lo + S);
+      ints[oGc + lane] = hi - lo;
+      for (int e = lo; e < hi; ++e) {
+        gr.push_back(bysp[k][e].first);
+        gnu.push_back(bysp[k][e].second);
+      }
+    }
+  }
+  const size_t oGr = ialloc(std::max<size_t>(gr.size(), 1)), oGnu = dalloc(std::max<size_t>(gnu.size(), 1));
+  std::copy(gr.begin(), gr.end(), ints.begin() + oGr);
+  std::copy(gnu.begin(), gnu.end(), dbl.begin() + oGnu);
+
+  const size_t dbytes = dbl.size() * sizeof(double), ibytes = ints.size() * sizeof(int);
+  HIP_TRY(hipMalloc(&c->d_tables, dbytes + ibytes + 256));
+  char* base = (char*)c->d_tables;
+  double* dd = (double*)base;
+  int* di = (int*)(base + ((dbytes + 255) / 256) * 256);
+  HIP_TRY(hipMemcpy(dd, dbl.data(), dbytes, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(di, ints.data(), ibytes, hipMemcpyHostToDevice));
+
+  DevMech& m = c->dm;
+  m.nsp = nsp;
+  m.nrxn = nr;
+  m.nfo = std::max(nfo, 1);
+  m.nthird = t->n_third;
+  m.rounds = (nr + 63) / 64;
+  m.gmax = S;
+  m.smax = smax;
+  m.tmid = t->tmid;
+  m.W = dd + oW;
+  m.invW = dd + oiW;
+  m.thermo = dd + oTh;
+  m.effT = dd + oEff;
+  m.arr = dd + oArr;
+  m.fo = dd + oFo;
+  m.g_nu = dd + oGnu;
+  m.meta = di + oMeta;
+  m.sp = di + oSp;
+  m.fo_nfc = di + oFoN;
+  m.g_start = di + oGs;
+  m.g_cnt = di + oGc;
+  m.s_first = di + oSf;
+  m.s_n = di + oSn;
+  m.g_r = di + oGr;
+
+  tgr_mech_info& in = c->info;
+  in.nsp = nsp;
+  in.nrxn = nr;
+  in.nslot = t->nslot;
+  in.n_third = t->n_third;
+  in.n_falloff = nfo;
+  in.n_nu = t->n_nu;
+  in.n_third_body_rxn = 0;
+  in.n_reversible = 0;
+  in.n_fc_terms = 0;
+  for (int r = 0; r < nr; ++r) {
+    in.n_third_body_rxn += t->rxn[r].kind == 1;
+    in.n_reversible += t->rxn[r].rev;
+    if (t->rxn[r].kind == 2) in.n_fc_terms += t->rxn[r].nfc;
+  }
+  return 0;
+}
+
+int species_index(const tgr_mech_tables* t, const char* name) {
+  for (int k = 0; k < t->nsp; ++k)
+    if (!strcmp(t->species[k], name)) return k;
+  return -1;
+}
+
+int check_grid(const tgr_grid* g, int nsp) {
+  if (!g) return fail(TGR_E_ARG, "grid descriptor is NULL");
+  if (g->imin_ < g->imino_ || g->imax_ > g->imaxo_ || g->jmin_ < g->jmino_ || g->jmax_ > g->jmaxo_ ||
+      g->kmin_ < g->kmino_ || g->kmax_ > g->kmaxo_)
+    return fail(TGR_E_ARG, "interior extents exceed array extents");
+  if (g->isc_sc < 1 || g->isc_sc + nsp - 1 > g->nscalar || g->isc_T < 1 || g->isc_T > g->nscalar)
+    return fail(TGR_E_ARG, "scalar indices out of range");
+  if (g->isc_T >= g->isc_sc && g->isc_T <= g->isc_sc + nsp - 1)
+    return fail(TGR_E_ARG, "isc_T overlaps the species block");
+  return 0;
+}
+
+int run_step(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const tgr_grid* g, double* rho,
+             double* sc, int isc_sc0, int isc_T0, const int* mask, double* hr, double* chem, hipStream_t s,
+             tgr_stats* stats) {
+  if (!(dt > 0.0) || !std::isfinite(dt)) return fail(TGR_E_ARG, "dt must be positive and finite");
+  const int nx = g->imax_ - g->imin_ + 1, ny = g->jmax_ - g->jmin_ + 1, nz = g->kmax_ - g->kmin_ + 1;
+  if (stats) memset(stats, 0, sizeof(*stats));
+  if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
+  const int64_t nxo = g->imaxo_ - g->imino_ + 1, nyo = g->jmaxo_ - g->jmino_ + 1, nzo = g->kmaxo_ - g->kmino_ + 1;
+  StepParams p{};
+  p.m = c->dm;
+  p.dt = dt;
+  p.p_gp = 0.5 * (p_thermo + p_thermo_old);
+  p.rtol = c->rtol;
+  p.atol = c->atol;
+  p.max_steps = c->max_steps;
+  p.qmax = c->qmax;
+  p.nx = nx;
+  p.ny = ny;
+  p.nz = nz;
+  p.sx = 1;
+  p.sy = nxo;
+  p.sz = nxo * nyo;
+  p.base = (int64_t)(g->imin_ - g->imino_) + (int64_t)(g->jmin_ - g->jmino_) * nxo +
+           (int64_t)(g->kmin_ - g->kmino_) * nxo * nyo;
+  p.plane = nxo * nyo * nzo;
+  p.isc_sc0 = isc_sc0;
+  p.isc_T0 = isc_T0;
+  p.mask = mask;
+  p.mask_ld = g->imaxo - g->imino + 1;
+  p.mask_base = (int64_t)(g->imin_ - g->imino) + (int64_t)(g->jmin_ - g->jmino) * p.mask_ld;
+  p.rho = rho;
+  p.sc = sc;
+  p.hr = hr;
+  p.chemsrc = chem;
+  p.idx_h2o = species_index(c->t, "H2O");
+  p.idx_co = species_index(c->t, "CO");
+  p.idx_co2 = species_index(c->t, "CO2");
+  p.ncell = (int64_t)nx * ny * nz;
+  p.totals = c->d_totals;
+  p.cell_stats = nullptr;
+  int grid;
+  if (c->persistent) {
+    p.queue = c->d_queue;
+    int per_cu = 8;
+    if (c->grid_override <= 0) {
+      int nb = tgr_step_occupancy(c->neq);
+      if (nb > 0) per_cu = nb;
+    }
+    int64_t gs = c->grid_override > 0 ? c->grid_override : (int64_t)c->n_cu * per_cu;
+    grid = (int)std::max<int64_t>(1, std::min<int64_t>(gs, p.ncell));
+  } else {
+    p.queue = nullptr;
+    if (p.ncell > 0x7fffffff) return fail(TGR_E_ARG, "too many cells for one launch");
+    grid = (int)p.ncell;
+  }
+  HIP_TRY(hipMemsetAsync(c->d_totals, 0, sizeof(unsigned long long) * TGR_NCOUNT, s));
+  HIP_TRY(hipEventRecord(c->ev0, s));
+  int rc = tgr_launch_step(&p, c->neq, grid, s);
+  if (rc) return fail(TGR_E_HIP, "step kernel launch failed");
+  HIP_TRY(hipEventRecord(c->ev1, s));
+  if (stats) {
+    unsigned long long tot[TGR_NCOUNT];
+    HIP_TRY(hipMemcpyAsync(tot, c->d_totals, sizeof(tot), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    int64_t cells = 0;
+    if (mask) {
+      // active cells: those the kernel integrated (counted through nfe > 0 is
+      // not robust for failures), recomputed on the host from the mask
+      std::vector<int> hm((size_t)(g->imaxo - g->imino + 1) * (g->jmaxo - g->jmino + 1));
+      HIP_TRY(hipMemcpy(hm.data(), mask, hm.size() * sizeof(int), hipMemcpyDefault));
+      for (int j = g->jmin_; j <= g->jmax_; ++j)
+        for (int i = g->imin_; i <= g->imax_; ++i)
+          cells += hm[(size_t)(j - g->jmino) * p.mask_ld + (i - g->imino)] != 1;
+      cells *= nz;
+    } else {
+      cells = p.ncell;
+    }
+    stats->cells = cells;
+    stats->nst = tot[0];
+    stats->nfe = tot[1];
+    stats->nfe_dq = tot[2];
+    stats->nje = tot[3];
+    stats->nsetups = tot[4];
+    stats->nni = tot[5];
+    stats->netf = tot[6];
+    stats->ncfn = tot[7];
+    stats->nlu = tot[8];
+    stats->failed = tot[9];
+    stats->kernel_ms = ms;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* tgr_chem_last_error(void) { return g_err.c_str(); }
+
+int tgr_chem_create(const char* mech, double rtol, double atol, int max_steps, tgr_chem** out) {
+  if (!out) return fail(TGR_E_ARG, "out is NULL");
+  *out = nullptr;
+  const tgr_mech_tables* t = find_mech(mech);
+  if (!t) return fail(TGR_E_MECH, std::string("unknown mechanism: ") + (mech ? mech : "(null)"));
+  if (!(rtol > 0.0) || !(atol > 0.0)) return fail(TGR_E_ARG, "tolerances must be positive");
+  const int neq = t->nsp + 1;
+  if (neq != 10 && neq != 37 && neq != 40) return fail(TGR_E_MECH, "no kernel instance for this size");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(TGR_E_HIP, "no HIP device");
+  tgr_chem* c = new tgr_chem();
+  c->t = t;
+  c->nsp = t->nsp;
+  c->neq = neq;
+  c->rtol = rtol;
+  c->atol = atol;
+  c->max_steps = max_steps > 0 ? max_steps : 10000;
+  int rc = 0;
+  if (hipGetDevice(&c->device) != hipSuccess) rc = fail(TGR_E_HIP, "hipGetDevice failed");
+  hipDeviceProp_t prop;
+  if (!rc && hipGetDeviceProperties(&prop, c->device) != hipSuccess) rc = fail(TGR_E_HIP, "device query failed");
+  if (!rc) c->n_cu = prop.multiProcessorCount;
+  if (!rc) rc = build_tables(c);
+  if (!rc && hipMalloc((void**)&c->d_queue, 256) != hipSuccess) rc = fail(TGR_E_NOMEM, "queue alloc");
+  if (!rc && hipMalloc((void**)&c->d_totals, 256) != hipSuccess) rc = fail(TGR_E_NOMEM, "totals alloc");
+  if (!rc && (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess))
+    rc = fail(TGR_E_HIP, "event create");
+  if (rc) {
+    tgr_chem_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+int tgr_chem_destroy(tgr_chem* c) {
+  if (!c) return 0;
+  if (c->d_tables) (void)hipFree(c->d_tables);
+  if (c->d_queue) (void)hipFree(c->d_queue);
+  if (c->d_totals) (void)hipFree(c->d_totals);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  c->b_rho.release();
+  c->b_sc.release();
+  c->b_mask.release();
+  c->b_hr.release();
+  c->b_chem.release();
+  delete c;
+  return 0;
+}
+
+int tgr_chem_mech_info(const tgr_chem* c, tgr_mech_info* info) {
+  if (!c || !info) return fail(TGR_E_ARG, "NULL argument");
+  *info = c->info;
+  return 0;
+}
+
+int tgr_chem_species_name(const tgr_chem* c, int k, char* buf, int len) {
+  if (!c || !buf || len <= 0 || k < 0 || k >= c->nsp) return fail(TGR_E_ARG, "bad species query");
+  snprintf(buf, (size_t)len, "%s", c->t->species[k]);
+  return 0;
+}
+
+int tgr_chem_molar_mass(const tgr_chem* c, double* W) {
+  if (!c || !W) return fail(TGR_E_ARG, "NULL argument");
+  memcpy(W, c->t->W, sizeof(double) * c->nsp);
+  return 0;
+}
+
+int tgr_chem_set_option(tgr_chem* c, const char* key, int64_t v) {
+  if (!c || !key) return fail(TGR_E_ARG, "NULL argument");
+  if (!strcmp(key, "persistent")) c->persistent = v != 0;
+  else if (!strcmp(key, "grid")) c->grid_override = v;
+  else if (!strcmp(key, "max_order")) {
+    if (v < 1 || v > 5) return fail(TGR_E_ARG, "max_order must be 1..5");
+    c->qmax = (int)v;
+  } else if (!strcmp(key, "max_steps")) {
+    if (v < 1) return fail(TGR_E_ARG, "max_steps must be positive");
+    c->max_steps = (int)v;
+  } else {
+    return fail(TGR_E_ARG, std::string("unknown option: ") + key);
+  }
+  return 0;
+}
+
+int tgr_chem_source_chemistry_device(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const tgr_grid* g,
+                                     double* RHO_, double* SC_, const int* mask, double* HR_gp, double* chemsrc_gp,
+                                     void* stream, tgr_stats* stats) {
+  if (!c || !RHO_ || !SC_) return fail(TGR_E_ARG, "NULL argument");
+  int rc = check_grid(g, c->nsp);
+  if (rc) return rc;
+  return run_step(c, dt, p_thermo, p_thermo_old, g, RHO_, SC_, g->isc_sc - 1, g->isc_T - 1, mask, HR_gp,
+                  chemsrc_gp, (hipStream_t)stream, stats);
+}
+
+int tgr_chem_source_chemistry(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const tgr_grid* g,
+                              double* RHO_, double* SC_, const int* mask, double* HR_gp, double* chemsrc_gp,
+                              tgr_stats* stats) {
+  if (!c || !RHO_ || !SC_) return fail(TGR_E_ARG, "NULL argument");
+  int rc = check_grid(g, c->nsp);
+  if (rc) return rc;
+  const int nsp = c->nsp;
+  const size_t plane = (size_t)(g->imaxo_ - g->imino_ + 1) * (g->jmaxo_ - g->jmino_ + 1) * (g->kmaxo_ - g->kmino_ + 1);
+  const size_t pb = plane * sizeof(double);
+  const size_t mbytes = (size_t)(g->imaxo - g->imino + 1) * (g->jmaxo - g->jmino + 1) * sizeof(int);
+  if ((rc = c->b_rho.ensure(pb)) || (rc = c->b_sc.ensure(pb * (nsp + 1)))) return rc;
+  if (mask && (rc = c->b_mask.ensure(mbytes))) return rc;
+  if (HR_gp && (rc = c->b_hr.ensure(pb))) return rc;
+  if (chemsrc_gp && (rc = c->b_chem.ensure(3 * pb))) return rc;
+  double* d_rho = (double*)c->b_rho.p;
+  double* d_sc = (double*)c->b_sc.p;
+  hipStream_t s = nullptr;
+  // species block -> planes 0..nsp-1, T -> plane nsp
+  HIP_TRY(hipMemcpyAsync(d_rho, RHO_, pb, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(d_sc, SC_ + (size_t)(g->isc_sc - 1) * plane, pb * nsp, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(d_sc + (size_t)nsp * plane, SC_ + (size_t)(g->isc_T - 1) * plane, pb,
+                         hipMemcpyHostToDevice, s));
+  if (mask) HIP_TRY(hipMemcpyAsync(c->b_mask.p, mask, mbytes, hipMemcpyHostToDevice, s));
+  if (HR_gp) HIP_TRY(hipMemcpyAsync(c->b_hr.p, HR_gp, pb, hipMemcpyHostToDevice, s));
+  if (chemsrc_gp) HIP_TRY(hipMemcpyAsync(c->b_chem.p, chemsrc_gp, 3 * pb, hipMemcpyHostToDevice, s));
+  tgr_stats local;
+  rc = run_step(c, dt, p_thermo, p_thermo_old, g, d_rho, d_sc, 0, nsp, mask ? (const int*)c->b_mask.p : nullptr,
+                HR_gp ? (double*)c->b_hr.p : nullptr, chemsrc_gp ? (double*)c->b_chem.p : nullptr, s,
+                stats ? stats : &local);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(SC_ + (size_t)(g->isc_sc - 1) * plane, d_sc, pb * nsp, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(SC_ + (size_t)(g->isc_T - 1) * plane, d_sc + (size_t)nsp * plane, pb,
+                         hipMemcpyDeviceToHost, s));
+  if (HR_gp) HIP_TRY(hipMemcpyAsync(HR_gp, c->b_hr.p, pb, hipMemcpyDeviceToHost, s));
+  if (chemsrc_gp) HIP_TRY(hipMemcpyAsync(chemsrc_gp, c->b_chem.p, 3 * pb, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+int tgr_chem_prodrates_device(tgr_chem* c, int64_t n, const double* C, const double* T, const double* P,
+                              double* cdot, void* stream) {
+  if (!c || (n > 0 && (!C || !T || !P || !cdot))) return fail(TGR_E_ARG, "NULL argument");
+  if (n < 0 || n > 0x7fffffff) return fail(TGR_E_ARG, "bad n");
+  if (tgr_launch_prodrates(&c->dm, n, C, T, P, cdot, (hipStream_t)stream)) return fail(TGR_E_HIP, "launch");
+  return 0;
+}
+
+int tgr_chem_rhs_device(tgr_chem* c, int64_t n, double p_gp, const double* sol, double* rhs, void* stream) {
+  if (!c || (n > 0 && (!sol || !rhs))) return fail(TGR_E_ARG, "NULL argument");
+  if (n < 0 || n > 0x7fffffff) return fail(TGR_E_ARG, "bad n");
+  if (tgr_launch_rhs(&c->dm, n, p_gp, sol, rhs, (hipStream_t)stream)) return fail(TGR_E_HIP, "launch");
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,268 @@
+"""Python host mirror of the reference's finite-rate chemistry interface.
+
+Wraps the C ABI of ``libtiger_chem.so`` (include/tiger_chem.h) with the names
+and argument meaning of the reference Fortran (combustion/finitechem.f90):
+
+* ``FiniteChem(mech, rtol, atol)``          ~ finitechem_init (184-319)
+* ``.source_chemistry(RHO_, SC_, ...)``    ~ finitechem_source_chemistry(RHO_, SC_) (962-1071)
+* ``.prodrates(C, T, P)``                  ~ PRODRATES (HYDROGENF.f90:4-277), batched
+* ``.compute_rhs(sol, p_gp)``              ~ finitechem_compute_rhs (93-175), batched
+* ``.species_names`` / ``.molar_mass``     ~ GETSPECIESNAMES / GETMOLARMASS
+
+Arrays are Fortran-ordered 3-D/4-D fields (i fastest).  Host numpy arrays
+take the synchronous host entry point; CUDA/HIP torch tensors take the
+device-resident entry point.  The HIP library is required: there is no CPU
+fallback, and a missing library raises ``RuntimeError``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtiger_chem.so")
+
+
+class _Grid(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "imin_", "imax_", "jmin_", "jmax_", "kmin_", "kmax_",
+        "imino_", "imaxo_", "jmino_", "jmaxo_", "kmino_", "kmaxo_",
+        "imino", "imaxo", "jmino", "jmaxo", "nscalar", "isc_sc", "isc_T")]
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in (
+        "cells", "nst", "nfe", "nfe_dq", "nje", "nsetups", "nni", "netf", "ncfn", "nlu", "failed")] + [
+        ("kernel_ms", ctypes.c_double)]
+
+
+class _Info(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "nsp", "nrxn", "nslot", "n_third", "n_falloff", "n_third_body_rxn", "n_reversible", "n_nu", "n_fc_terms")]
+
+
+@dataclass
+class Grid:
+    """Block extents (reference geometry/partition.f90:53-90, masks.f90:55);
+    scalar indices are 1-based as in the reference."""
+    imin_: int
+    imax_: int
+    jmin_: int
+    jmax_: int
+    kmin_: int
+    kmax_: int
+    imino_: int
+    imaxo_: int
+    jmino_: int
+    jmaxo_: int
+    kmino_: int
+    kmaxo_: int
+    nscalar: int
+    isc_sc: int
+    isc_T: int
+    imino: int | None = None
+    imaxo: int | None = None
+    jmino: int | None = None
+    jmaxo: int | None = None
+
+    @staticmethod
+    def block(nx, ny, nz, nscalar, isc_sc, isc_T, nover=0):
+        """Interior 1..n with ``nover`` ghost layers on each side."""
+        return Grid(1, nx, 1, ny, 1, nz, 1 - nover, nx + nover, 1 - nover, ny + nover, 1 - nover, nz + nover,
+                    nscalar, isc_sc, isc_T)
+
+    @property
+    def shape(self):
+        return (self.imaxo_ - self.imino_ + 1, self.jmaxo_ - self.jmino_ + 1, self.kmaxo_ - self.kmino_ + 1)
+
+    @property
+    def mask_shape(self):
+        return (self.imaxo_m - self.imino_m + 1, self.jmaxo_m - self.jmino_m + 1)
+
+    @property
+    def imino_m(self):
+        return self.imino_ if self.imino is None else self.imino
+
+    @property
+    def imaxo_m(self):
+        return self.imaxo_ if self.imaxo is None else self.imaxo
+
+    @property
+    def jmino_m(self):
+        return self.jmino_ if self.jmino is None else self.jmino
+
+    @property
+    def jmaxo_m(self):
+        return self.jmaxo_ if self.jmaxo is None else self.jmaxo
+
+    def _c(self):
+        return _Grid(self.imin_, self.imax_, self.jmin_, self.jmax_, self.kmin_, self.kmax_,
+                     self.imino_, self.imaxo_, self.jmino_, self.jmaxo_, self.kmino_, self.kmaxo_,
+                     self.imino_m, self.imaxo_m, self.jmino_m, self.jmaxo_m,
+                     self.nscalar, self.isc_sc, self.isc_T)
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load the HIP chemistry library; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("libtiger_chem.so not built (%s): run __graft_entry__.build()" % path)
+    lib = ctypes.CDLL(path)
+    vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    lib.tgr_chem_create.argtypes = [ctypes.c_char_p, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                    ctypes.POINTER(vp)]
+    lib.tgr_chem_destroy.argtypes = [vp]
+    lib.tgr_chem_mech_info.argtypes = [vp, ctypes.POINTER(_Info)]
+    lib.tgr_chem_species_name.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    lib.tgr_chem_molar_mass.argtypes = [vp, dp]
+    step_args = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.POINTER(_Grid), vp, vp, vp, vp, vp]
+    lib.tgr_chem_source_chemistry.argtypes = step_args + [ctypes.POINTER(_Stats)]
+    lib.tgr_chem_source_chemistry_device.argtypes = step_args + [vp, ctypes.POINTER(_Stats)]
+    lib.tgr_chem_prodrates_device.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp, vp]
+    lib.tgr_chem_rhs_device.argtypes = [vp, ctypes.c_int64, ctypes.c_double, vp, vp, vp]
+    lib.tgr_chem_set_option.argtypes = [vp, ctypes.c_char_p, ctypes.c_int64]
+    lib.tgr_chem_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+class ChemError(RuntimeError):
+    pass
+
+
+def _check(lib, rc, what):
+    if rc != 0:
+        raise ChemError("%s failed (%d): %s" % (what, rc, lib.tgr_chem_last_error().decode()))
+
+
+def _ptr(a):
+    """Raw pointer of a numpy array or torch tensor (None -> NULL)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()
+
+
+def _is_device(a):
+    return a is not None and not isinstance(a, np.ndarray) and getattr(a, "is_cuda", False)
+
+
+class FiniteChem:
+    """Finite-rate chemistry on one GPU (mechanism: 'h2' | 'gri30' | 'dme39')."""
+
+    def __init__(self, mech: str, rtol: float = 1e-6, atol: float = 1e-12, max_steps: int = 10000):
+        self.lib = load_library()
+        self.mech = mech
+        self.rtol, self.atol = rtol, atol
+        h = ctypes.c_void_p()
+        _check(self.lib, self.lib.tgr_chem_create(mech.encode(), rtol, atol, max_steps, ctypes.byref(h)),
+               "tgr_chem_create")
+        self._h = h
+        info = _Info()
+        _check(self.lib, self.lib.tgr_chem_mech_info(h, ctypes.byref(info)), "mech_info")
+        self.info = {n: getattr(info, n) for n, _ in _Info._fields_}
+        self.nsp = info.nsp
+        self.neq = info.nsp + 1
+        buf = ctypes.create_string_buffer(64)
+        names = []
+        for k in range(self.nsp):
+            _check(self.lib, self.lib.tgr_chem_species_name(h, k, buf, 64), "species_name")
+            names.append(buf.value.decode())
+        self.species_names = names
+        W = np.zeros(self.nsp)
+        _check(self.lib, self.lib.tgr_chem_molar_mass(h, W.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+               "molar_mass")
+        self.molar_mass = W
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.tgr_chem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_option(self, key: str, value: int):
+        _check(self.lib, self.lib.tgr_chem_set_option(self._h, key.encode(), int(value)), "set_option")
+
+    def source_chemistry(self, RHO_, SC_, grid: Grid, dt: float, p_thermo: float, p_thermo_old: float | None = None,
+                         mask=None, HR_gp=None, chemsrc_gp=None, stream=None, sync: bool = True):
+        """Advance species and T of the interior, non-wall cells by dt in place
+        (finitechem.f90:962-1071).  RHO_: (nxo,nyo,nzo) and SC_: (nxo,nyo,nzo,nscalar)
+        Fortran-ordered float64; mask: (mx,my) int32 Fortran-ordered (1 = wall)."""
+        if p_thermo_old is None:
+            p_thermo_old = p_thermo
+        dev = _is_device(SC_)
+        self._check_arrays(RHO_, SC_, grid, mask, HR_gp, chemsrc_gp, dev)
+        st = _Stats()
+        g = grid._c()
+        if dev:
+            rc = self.lib.tgr_chem_source_chemistry_device(
+                self._h, dt, p_thermo, p_thermo_old, ctypes.byref(g), _ptr(RHO_), _ptr(SC_), _ptr(mask),
+                _ptr(HR_gp), _ptr(chemsrc_gp), stream, ctypes.byref(st) if sync else None)
+        else:
+            rc = self.lib.tgr_chem_source_chemistry(
+                self._h, dt, p_thermo, p_thermo_old, ctypes.byref(g), _ptr(RHO_), _ptr(SC_), _ptr(mask),
+                _ptr(HR_gp), _ptr(chemsrc_gp), ctypes.byref(st))
+        _check(self.lib, rc, "source_chemistry")
+        return {n: getattr(st, n) for n, _ in _Stats._fields_}
+
+    @staticmethod
+    def _fortran_ok(a, shape, dtype):
+        if isinstance(a, np.ndarray):
+            return a.shape == shape and a.dtype == dtype and a.flags.f_contiguous
+        # torch tensor: Fortran order means stride[0] == 1 and increasing strides
+        st = a.stride()
+        exp, acc = [], 1
+        for s in shape:
+            exp.append(acc)
+            acc *= s
+        tdt = {np.float64: "torch.float64", np.int32: "torch.int32"}[dtype]
+        return tuple(a.shape) == shape and tuple(st) == tuple(exp) and str(a.dtype) == tdt
+
+    def _check_arrays(self, RHO_, SC_, grid, mask, HR_gp, chemsrc_gp, dev):
+        sh = grid.shape
+        if not self._fortran_ok(RHO_, sh, np.float64):
+            raise ValueError("RHO_ must be float64 Fortran-ordered with shape %s" % (sh,))
+        if not self._fortran_ok(SC_, sh + (grid.nscalar,), np.float64):
+            raise ValueError("SC_ must be float64 Fortran-ordered with shape %s" % (sh + (grid.nscalar,),))
+        if mask is not None and not self._fortran_ok(mask, grid.mask_shape, np.int32):
+            raise ValueError("mask must be int32 Fortran-ordered with shape %s" % (grid.mask_shape,))
+        if HR_gp is not None and not self._fortran_ok(HR_gp, sh, np.float64):
+            raise ValueError("HR_gp has the wrong shape/layout")
+        if chemsrc_gp is not None and not self._fortran_ok(chemsrc_gp, sh + (3,), np.float64):
+            raise ValueError("chemsrc_gp has the wrong shape/layout")
+        for a in (RHO_, SC_, mask, HR_gp, chemsrc_gp):
+            if a is not None and _is_device(a) != dev:
+                raise ValueError("mix of host and device arrays")
+
+    def prodrates(self, C, T, P, stream=None):
+        """CDOT[nsp, n] from C[nsp, n] (kmol/m3), T[n], P[n] -- device tensors."""
+        import torch
+        n = T.shape[0]
+        assert C.shape == (self.nsp, n) and C.is_contiguous() and T.is_contiguous() and P.is_contiguous()
+        out = torch.empty_like(C)
+        _check(self.lib, self.lib.tgr_chem_prodrates_device(self._h, n, _ptr(C), _ptr(T), _ptr(P), _ptr(out),
+                                                            stream), "prodrates")
+        return out
+
+    def compute_rhs(self, sol, p_gp: float, stream=None):
+        """rhs[neq, n] of sol[neq, n] = [rho*Y, rho*T] -- device tensors."""
+        import torch
+        n = sol.shape[1]
+        assert sol.shape == (self.neq, n) and sol.is_contiguous()
+        out = torch.empty_like(sol)
+        _check(self.lib, self.lib.tgr_chem_rhs_device(self._h, n, p_gp, _ptr(sol), _ptr(out), stream), "rhs")
+        return out

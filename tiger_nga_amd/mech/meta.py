@@ -1,0 +1,71 @@
+"""Species metadata of the compiled mechanisms, read from the generated C
+tables (``tiger_nga_amd/csrc/mech_<name>.h``) so that host code needs no GPU.
+
+Element compositions come from the species formulas; the aliases are the
+ones the reference lists for DME39 (combustion/DME39F.f90:18-23) plus the
+singlet-CH2 spellings of GRI-3.0 ('CH2*-CH2') and DME39 ('CH2X').
+"""
+from __future__ import annotations
+
+import functools
+import os
+import re
+
+import numpy as np
+
+_CSRC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "csrc")
+MECHS = ("h2", "gri30", "dme39")
+ELEMENTS = ("C", "H", "O", "N", "AR")
+
+ALIASES = {
+    "CH2*-CH2": "CH2", "CH2X": "CH2",
+    "RO2": "CH3OCH2O2", "ROH": "CH3OCH2OH", "QOOH": "CH2OCH2O2H",
+    "O2QOOH": "O2CH2OCH2O2H", "HO2QHO": "HO2CH2OCHO",
+}
+
+
+@functools.lru_cache(maxsize=None)
+def _header(mech: str) -> str:
+    with open(os.path.join(_CSRC, "mech_%s.h" % mech)) as f:
+        return f.read()
+
+
+@functools.lru_cache(maxsize=None)
+def species(mech: str) -> tuple:
+    m = re.search(r"%s_species\[\d+\] = \{([^}]*)\}" % mech, _header(mech))
+    return tuple(s.strip().strip('"') for s in m.group(1).split(","))
+
+
+@functools.lru_cache(maxsize=None)
+def molar_mass(mech: str) -> np.ndarray:
+    """GETMOLARMASS values (binary32 literals widened), kg/kmol."""
+    m = re.search(r"%s_W\[\d+\] = \{([^}]*)\}" % mech, _header(mech))
+    return np.array([float.fromhex(x.strip()) for x in m.group(1).split(",")])
+
+
+def composition(name: str) -> dict:
+    if name == "AR":
+        return {"AR": 1}
+    f = ALIASES.get(name, name)
+    out = {}
+    for el, n in re.findall(r"(C|H|O|N)(\d*)", f):
+        out[el] = out.get(el, 0) + (int(n) if n else 1)
+    rebuilt = "".join(el + (n if n else "") for el, n in re.findall(r"(C|H|O|N)(\d*)", f))
+    if rebuilt != f:
+        raise ValueError("cannot parse formula %r" % name)
+    return out
+
+
+@functools.lru_cache(maxsize=None)
+def element_matrix(mech: str) -> np.ndarray:
+    """E[e, k] = atoms of element e in species k (ELEMENTS order)."""
+    sp = species(mech)
+    E = np.zeros((len(ELEMENTS), len(sp)))
+    for k, s in enumerate(sp):
+        for el, n in composition(s).items():
+            E[ELEMENTS.index(el), k] = n
+    return E
+
+
+def index(mech: str, name: str) -> int:
+    return species(mech).index(name)
