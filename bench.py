@@ -1,0 +1,158 @@
+#!/usr/bin/env python3
+"""Chemistry-step benchmark on one MI355X.
+
+Workload (default ``gri30``): GRI-Mech 3.0 as shipped with the reference
+(grimech30F, 36 species, 219 reactions / 422 rate slots) on 64^3 = 262,144
+cells at 1 atm, dt = 5e-7 s, seeded synthetic states (tiger_nga_amd/inputs.py),
+plain FP64 path at the reference tolerances (rtol 1e-6, atol 1e-12,
+MAX_NSTEPS 10000).  One step = one finitechem_source_chemistry call over all
+cells with the fields resident in HBM (device entry point); before each step
+the fields are restored from a pristine device copy so that every step does
+the same work.
+
+Prints one JSON line: cell-steps/s, the FP64 roofline of the step kernel
+(algorithmic flops from the integrator counters / kernel time, HIP events on
+the launch stream) and the CPU baseline: the C oracle (CVODE-semantics BDF and
+table-driven mechanism) on a bounded sample of the same cells, one cell loop
+per host thread.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+PEAK_FP64 = 78.6e12
+PEAK_HBM = 8.0e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="gri30", choices=["h2", "gri30", "gri30_uneven", "dme39"])
+    ap.add_argument("--cpu-cells", type=int, default=8192)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: all usable host cores, at most 16")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def fortran_empty(torch, shape, dev):
+    strides, acc = [], 1
+    for s in shape:
+        strides.append(acc)
+        acc *= s
+    return torch.empty_strided(shape, tuple(strides), dtype=torch.float64, device=dev)
+
+
+def cpu_baseline(workload, ncell, nthreads):
+    import numpy as np
+
+    from oracle import pyoracle
+    from tiger_nga_amd import inputs
+    mech, Y, T, rho, P, dt = inputs.states(workload, n=ncell, seed=0)
+    sol = inputs.packed_state(Y, T, rho)
+    t0 = time.perf_counter()
+    pyoracle.integrate_batch(mech, sol, dt, P, rtol=1e-6, atol=1e-12, nthreads=nthreads)
+    el = time.perf_counter() - t0
+    n1 = max(16, ncell // 64)
+    t0 = time.perf_counter()
+    pyoracle.integrate_batch(mech, sol[:n1], dt, P, rtol=1e-6, atol=1e-12, nthreads=1)
+    el1 = time.perf_counter() - t0
+    return {"value": ncell / el, "unit": "cell-steps/s", "cores": nthreads, "kind": "port",
+            "sample": "first %d cells of the %s workload (same seeded states), C oracle BDF at rtol 1e-6, "
+                      "%d threads x serial cell loop; %.2f s wall" % (ncell, workload, nthreads, el),
+            "single_core_value": n1 / el1}
+
+
+def main():
+    a = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: the chemistry step has no cross-GPU work in this build "
+              "(cells are independent); run with --gpus 1", file=sys.stderr)
+        return 2
+    import numpy as np
+    import torch
+
+    from tiger_nga_amd import flops, inputs
+    from tiger_nga_amd.finitechem import FiniteChem, Grid
+
+    dev = "cuda:0"
+    torch.cuda.set_device(0)
+    mech, Y, T, rho, P, dt = inputs.states(a.workload, seed=0)
+    n = Y.shape[0]
+    nx = int(round(n ** (1.0 / 3.0)))
+    assert nx ** 3 == n
+    RHO_, SC_, isc_sc, isc_T = inputs.fortran_fields(Y, T, rho, nx, nx, nx)
+    g = Grid.block(nx, nx, nx, SC_.shape[3], isc_sc, isc_T)
+    R_d = fortran_empty(torch, RHO_.shape, dev)
+    S0_d = fortran_empty(torch, SC_.shape, dev)
+    S_d = fortran_empty(torch, SC_.shape, dev)
+    HR_d = fortran_empty(torch, RHO_.shape, dev)
+    CS_d = fortran_empty(torch, RHO_.shape + (3,), dev)
+    R_d.copy_(torch.from_numpy(RHO_))
+    S0_d.copy_(torch.from_numpy(SC_))
+    del RHO_, SC_, Y, T, rho
+    fc = FiniteChem(mech, 1e-6, 1e-12, 10000)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+
+    stats = None
+    for _ in range(max(1, a.warmup)):
+        S_d.copy_(S0_d)
+        stats = fc.source_chemistry(R_d, S_d, g, dt, P, P, HR_gp=HR_d, chemsrc_gp=CS_d, stream=sp, sync=True)
+    torch.cuda.synchronize()
+
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        S_d.copy_(S0_d)
+        evs[k][0].record(stream)
+        fc.source_chemistry(R_d, S_d, g, dt, P, P, HR_gp=HR_d, chemsrc_gp=CS_d, stream=sp, sync=False)
+        evs[k][1].record(stream)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    kms = [e0.elapsed_time(e1) for e0, e1 in evs]
+    kernel_s = sum(kms) / len(kms) / 1e3
+
+    info = fc.info
+    F = flops.step_flops(info, stats)
+    achieved = F / kernel_s
+    cells = stats["cells"]
+    out = {
+        "metric": "GRI-3.0 chem cell-steps/s, 1 GPU: surrogate-guided vs plain FP64 at err<=1e-6",
+        "value": cells * a.steps / el,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": el / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic seeded states (no case data ships with the reference)",
+        "config": {"workload": "%s plain FP64 step, %d^3 = %d cells, dt %.1e s, P %.0f Pa, rtol 1e-6 atol 1e-12"
+                               % (a.workload, nx, n, dt, P), "mechanism": mech, "path": "plain_fp64",
+                   "species": info["nsp"], "reactions": info["nrxn"], "rate_slots": info["nslot"]},
+        "roofline": {"bound": "mfma", "compute_unit": "FP64 vector ALU (MI355X FP64 vector peak = FP64 matrix "
+                                                       "peak = 78.6 TFLOP/s)",
+                     "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64, "traffic": None,
+                     "kernel_ms": kernel_s * 1e3, "flops_per_step": F,
+                     "algorithmic_bytes_per_step": flops.step_bytes(info, cells)},
+        "solver": {k: stats[k] for k in ("nst", "nfe", "nfe_dq", "nje", "nsetups", "nni", "netf", "ncfn", "nlu",
+                                         "failed")},
+    }
+    if not a.no_cpu:
+        nt = a.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        out["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_cells, nt)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -54,7 +54,10 @@ typedef struct {
  * HYDROGENF.f90:408-450) plus the table sizes the kernels use. */
 typedef struct {
   int nsp, nrxn, nslot, n_third, n_falloff, n_third_body_rxn, n_reversible, n_nu;
-  int n_fc_terms; /* sum of Fc exponential terms over fall-off reactions */
+  int n_fc_terms;       /* sum of Fc exponential terms over fall-off reactions */
+  int n_conc_factors;   /* concentration factors over all rates of progress (forward + backward) */
+  int n_eff_corr;       /* third-body efficiencies different from 1 (terms beyond the total concentration) */
+  int n_rev_falloff;    /* reversible fall-off reactions */
 } tgr_mech_info;
 
 /* Create a context: mechanism tables on the current HIP device, solver
@@ -103,8 +106,8 @@ int tgr_chem_prodrates_device(tgr_chem *ctx, int64_t n, const double *C, const d
 int tgr_chem_rhs_device(tgr_chem *ctx, int64_t n, double p_gp, const double *sol, double *rhs, void *stream);
 
 /* Tuning knobs (no counterpart in the reference):
- *   "persistent"  1: persistent grid pulling cells from a device counter (default), 0: one block per cell
- *   "grid"        persistent grid size in blocks (0: occupancy x CUs)
+ *   "grid"        size of the persistent grid in blocks (0: occupancy x CUs); blocks pull cells
+ *                 from a device work counter
  *   "max_order"   BDF maximum order 1..5 (CVODE default 5)
  *   "max_steps"   MAX_NSTEPS */
 int tgr_chem_set_option(tgr_chem *ctx, const char *key, int64_t value);

@@ -104,7 +104,9 @@ def test_step_vs_oracle(mech, workload, rtol, atol, ytol, ttol, chem, oracle, to
 
     Tight tolerance (parity mode): mass fractions to rel 1e-8 (floor 1e-12)
     and T to 1e-6 K.  Reference tolerance: both run the same CVODE-semantics
-    BDF, so most cells agree far below rtol; bounded by a few rtol overall."""
+    BDF; rounding differences move Newton iterates by fractions of the
+    tolerance, so the bar is the solver's own error weight:
+    |d(rho Y)| / (rtol |rho Y| + atol) <= 1 and |dT| <= 1e-4 K."""
     nx, ny, nz = 4, 4, 4
     g, RHO_, SC_, mask, P, dt = _grid_case(mech, workload, nx, ny, nz, 2, 11, True)
     fc = chem(mech, rtol, atol)
@@ -132,7 +134,10 @@ def test_step_vs_oracle(mech, workload, rtol, atol, ytol, ttol, chem, oracle, to
     assert np.array_equal(SC_g[wall], SC_[wall]) and np.all(HR_g[wall] == 3.0)
     assert st["failed"] == 0 and sto["status"] == 0
     assert st["cells"] == nx * ny * nz - ny * nz
+    rho = RHO_[..., None]
+    ew = np.abs(rho * (SC_g[..., sp] - SC_o[..., sp])) / (rtol * np.abs(rho * SC_o[..., sp]) + atol)
+    print("   max weighted err %.3e" % ew.max())
     if ytol is not None:
         assert eY.max() < ytol and eT.max() < ttol
     else:
-        assert eY.max() < 50 * rtol and eT.max() < 1e-3
+        assert ew.max() <= 1.0 and eT.max() < 1e-4

@@ -7,8 +7,10 @@
 // integrator follows the published CVODE algorithm step for step (the same
 // restatement the CPU oracle holds), laid out for a 64-wide wavefront:
 //   * vectors of length neq = nsp+1 live one component per lane;
-//   * the saved Jacobian J and the iteration matrix M = I - gamma*J live one
-//     ROW per lane in registers (NEQ doubles each);
+//   * the iteration matrix M = I - gamma*J lives one ROW per lane in
+//     registers (NEQ doubles); the saved Jacobian J (reused for up to 50
+//     steps) lives in a per-wave HBM workspace, column-major with the rows
+//     in lanes, so each column store / row load is one coalesced access;
 //   * LU with partial pivoting keeps rows in their lanes (implicit row
 //     exchange): the pivot is a wave arg-max, the pivot row is broadcast with
 //     v_readlane, and each lane records the elimination step of its row;
@@ -37,6 +39,9 @@ constexpr double UROUND = 2.2204460492503131e-16;
 enum { NF_FIRST = 0, NF_PREV_CONV = 1, NF_PREV_ERR = 2, NF_OK = 3, NF_CONV_FAIL = 4, NF_TRY_AGAIN = 5 };
 enum { CF_NONE = 0, CF_BAD_J = 1, CF_OTHER = 2 };
 
+// x^y for x >= 0 through the project's exp/log (step-size ratios only)
+__device__ __forceinline__ double rpow(double x, double y) { return fexp(y * flog(x)); }
+
 template <class T, int N>
 __device__ __forceinline__ T pick(const T (&a)[N], int i) {
   T r = a[0];
@@ -53,48 +58,58 @@ __device__ __forceinline__ void put(T (&a)[N], int i, T v) {
 
 template <int NEQ>
 struct Bdf {
-  const DevMech& m;
-  RhsLds& L;
+  const DevMech m;  // by value: fields stay in scalar registers
+  RhsLds<NEQ>& L;
   double p_gp, rtol, atol;
   int lane, n, qmax, max_steps;
   bool act;
-  // per-lane vectors
-  double zn[kQmaxAbs + 1];
+  // per-lane vectors; the Nordsieck history lives in LDS (lane-private
+  // columns), so that zn[q] with a run-time order q is a plain address
+  double (&zn)[kQmaxAbs + 2][kWave];
   double ewt, acor, y, tempv, ftemp;
-  double J[NEQ], M[NEQ];
+  double* jw;  // saved Jacobian: jw[c * 64 + lane] = J(lane, c)
+  double M[NEQ];
   int pstep;
   double rdiag;
   // wave-uniform control state
   double tn, h, hprime, hscale, eta, etamax, rl1, gamma, gammap, gamrat, crate, acnrm, saved_tq5;
-  double tq[6], l[kQmaxAbs + 1], tau[kQmaxAbs + 2];
+  // small wave-uniform arrays indexed by the run-time order (LDS, every lane
+  // writes the same value)
+  double* tq;
+  double* l;
+  double* tau;
   int q, qprime, qwait, Lq, jcur, nst, nstlp, nstlj;
   int c_nfe, c_nfe_dq, c_nje, c_nsetups, c_nni, c_netf, c_ncfn, c_nlu;
 
-  __device__ Bdf(const DevMech& mech, RhsLds& lds, double pgp, double rt, double at, int qm, int ms)
-      : m(mech), L(lds), p_gp(pgp), rtol(rt), atol(at), max_steps(ms) {
+  __device__ Bdf(const DevMech& mech, RhsLds<NEQ>& lds, double (&znl)[kQmaxAbs + 2][kWave], double* ws, double pgp,
+                 double rt, double at, int qm, int ms)
+      : m(mech), L(lds), p_gp(pgp), rtol(rt), atol(at), max_steps(ms), zn(znl), jw(ws) {
+    tq = &znl[kQmaxAbs + 1][0];
+    l = tq + 8;
+    tau = tq + 16;
     lane = lane_id();
     n = NEQ;
     act = lane < NEQ;
     qmax = (qm >= 1 && qm <= kQmaxAbs) ? qm : kQmaxAbs;
   }
 
-  __device__ double f(double yv) { return rhs_wave(m, L, act ? yv : 0.0, p_gp); }
+  __device__ __forceinline__ double f(double yv) { return rhs_wave<NEQ>(m, L, act ? yv : 0.0, p_gp); }
 
-  __device__ double wrms(double x) {
+  __device__ __forceinline__ double wrms(double x) {
     double v = act ? x * ewt : 0.0;
-    return sqrt(uni(wave_sum(v * v)) / n);
+    return sqrt((wave_sum(v * v)) / n);
   }
 
-  __device__ void set_ewt(double yv) { ewt = act ? 1.0 / (rtol * fabs(yv) + atol) : 0.0; }
+  __device__ __forceinline__ void set_ewt(double yv) { ewt = act ? 1.0 / (rtol * fabs(yv) + atol) : 0.0; }
 
   // ---- dense linear algebra, one matrix row per lane --------------------
-  __device__ int getrf() {
+  __device__ __forceinline__ int getrf() {
     pstep = act ? NEQ : -1;
 #pragma unroll
     for (int k = 0; k < NEQ; ++k) {
       const bool cand = (pstep == NEQ);
       const double a = cand ? fabs(M[k]) : -1.0;
-      const double amax = uni(wave_max(a));
+      const double amax = (wave_max(a));
       if (!(amax > 0.0)) return k + 1;
       const uint64_t bal = __ballot(cand && a == amax);
       const int p = __builtin_amdgcn_readfirstlane(bal ? (int)__builtin_ctzll(bal) : 0);
@@ -117,7 +132,7 @@ struct Bdf {
   }
 
   // solve M x = b with the factors from getrf; b, x: one component per lane
-  __device__ double getrs(double b) {
+  __device__ __forceinline__ double getrs(double b) {
     double xo = 0.0;
 #pragma unroll
     for (int k = 0; k < NEQ; ++k) {
@@ -138,7 +153,7 @@ struct Bdf {
   }
 
   // difference-quotient Jacobian, column j perturbs component j
-  __device__ void dq_jac(double yv, double fy) {
+  __device__ __forceinline__ void dq_jac(double yv, double fy) {
     const double srur = sqrt(UROUND);
     const double fnorm = wrms(fy);
     const double minInc = (fnorm != 0.0) ? (MIN_INC_MULT * fabs(h) * UROUND * n * fnorm) : 1.0;
@@ -150,13 +165,11 @@ struct Bdf {
       const double fp = f(yp);
       c_nfe_dq++;
       const double ii = 1.0 / inc;
-      const double col = ii * fp - ii * fy;
-#pragma unroll
-      for (int c = 0; c < NEQ; ++c) J[c] = (c == j) ? col : J[c];
+      fresh(jw)[j * kWave + lane] = ii * fp - ii * fy;
     }
   }
 
-  __device__ int lsetup(int convfail) {
+  __device__ __forceinline__ int lsetup(int convfail) {
     const double dgamma = fabs(gamma / gammap - 1.0);
     const bool jbad = (nst == 0) || (nst > nstlj + MSBJ) || ((convfail == CF_BAD_J) && (dgamma < DGMAX_J)) ||
                       (convfail == CF_OTHER);
@@ -166,11 +179,12 @@ struct Bdf {
       c_nje++;
       nstlj = nst;
       jcur = 1;
-      dq_jac(zn[0], ftemp);
+      dq_jac(zn[0][lane], ftemp);
     }
+    const double* jr = fresh(jw) + lane;
 #pragma unroll
     for (int c = 0; c < NEQ; ++c) {
-      double v = -gamma * J[c];
+      double v = -gamma * jr[c * kWave];
       if (c == lane) v += 1.0;
       M[c] = v;
     }
@@ -179,7 +193,7 @@ struct Bdf {
   }
 
   // ---- BDF bookkeeping ----------------------------------------------------
-  __device__ void set_bdf() {
+  __device__ __forceinline__ void set_bdf() {
     double xi_inv = 1.0, xistar_inv = 1.0, alpha0 = -1.0, alpha0_hat = -1.0, hsum = h;
     l[0] = l[1] = 1.0;
 #pragma unroll
@@ -197,7 +211,7 @@ struct Bdf {
       }
       alpha0 -= 1.0 / q;
       xistar_inv = -l[1] - alpha0;
-      hsum += pick(tau, q - 1);
+      hsum += tau[q - 1];
       xi_inv = h / hsum;
       alpha0_hat = -l[1] - xi_inv;
 #pragma unroll
@@ -206,7 +220,7 @@ struct Bdf {
     }
     const double A1 = 1.0 - alpha0_hat + alpha0;
     const double A2 = 1.0 + q * A1;
-    const double lq = pick(l, q);
+    const double lq = l[q];
     tq[2] = fabs(A1 / (alpha0 * A2));
     tq[5] = fabs(A2 * xistar_inv / (lq * xi_inv));
     if (qwait == 1) {
@@ -219,7 +233,7 @@ struct Bdf {
       } else {
         tq[1] = 1.0;
       }
-      hsum += pick(tau, q);
+      hsum += tau[q];
       xi_inv = h / hsum;
       const double A5 = alpha0 - (1.0 / (q + 1));
       const double A6 = alpha0_hat - xi_inv;
@@ -229,7 +243,7 @@ struct Bdf {
     tq[4] = NLSCOEF / tq[2];
   }
 
-  __device__ void set_coeffs() {
+  __device__ __forceinline__ void set_coeffs() {
     set_bdf();
     rl1 = 1.0 / l[1];
     gamma = h * rl1;
@@ -237,38 +251,38 @@ struct Bdf {
     gamrat = (nst > 0) ? gamma / gammap : 1.0;
   }
 
-  __device__ void predict(double tstop) {
+  __device__ __forceinline__ void predict(double tstop) {
     tn += h;
     if ((tn - tstop) * h > 0.0) tn = tstop;
 #pragma unroll
     for (int k = 1; k <= kQmaxAbs; ++k)
 #pragma unroll
       for (int j = kQmaxAbs; j >= k; --j)
-        if (k <= q && j <= q) zn[j - 1] += zn[j];
+        if (k <= q && j <= q) zn[j - 1][lane] += zn[j][lane];
   }
 
-  __device__ void restore(double saved_t) {
+  __device__ __forceinline__ void restore(double saved_t) {
     tn = saved_t;
 #pragma unroll
     for (int k = 1; k <= kQmaxAbs; ++k)
 #pragma unroll
       for (int j = kQmaxAbs; j >= k; --j)
-        if (k <= q && j <= q) zn[j - 1] -= zn[j];
+        if (k <= q && j <= q) zn[j - 1][lane] -= zn[j][lane];
   }
 
-  __device__ void rescale() {
+  __device__ __forceinline__ void rescale() {
     double factor = eta;
 #pragma unroll
     for (int j = 1; j <= kQmaxAbs; ++j)
       if (j <= q) {
-        zn[j] *= factor;
+        zn[j][lane] *= factor;
         factor *= eta;
       }
     h = hscale * eta;
     hscale = h;
   }
 
-  __device__ void increase_order() {
+  __device__ __forceinline__ void increase_order() {
     double lo[kQmaxAbs + 2];
 #pragma unroll
     for (int i = 0; i < kQmaxAbs + 2; ++i) lo[i] = 0.0;
@@ -288,14 +302,14 @@ struct Bdf {
       }
     }
     const double A1 = (-alpha0 - alpha1) / prod;
-    const double zl = A1 * pick(zn, qmax);
-    put(zn, Lq, zl);
+    const double zl = A1 * zn[qmax][lane];
+    zn[Lq][lane] = zl;
 #pragma unroll
     for (int j = 2; j <= kQmaxAbs; ++j)
-      if (j <= q) zn[j] += lo[j] * zl;
+      if (j <= q) zn[j][lane] += lo[j] * zl;
   }
 
-  __device__ void decrease_order() {
+  __device__ __forceinline__ void decrease_order() {
     double lo[kQmaxAbs + 2];
 #pragma unroll
     for (int i = 0; i < kQmaxAbs + 2; ++i) lo[i] = 0.0;
@@ -310,19 +324,19 @@ struct Bdf {
         for (int i = j + 2; i >= 2; --i) lo[i] = lo[i] * xi + lo[i - 1];
       }
     }
-    const double zq = pick(zn, q);
+    const double zq = zn[q][lane];
 #pragma unroll
     for (int j = 2; j < kQmaxAbs; ++j)
-      if (j < q) zn[j] -= lo[j] * zq;
+      if (j < q) zn[j][lane] -= lo[j] * zq;
   }
 
-  __device__ void adjust_order(int dq) {
+  __device__ __forceinline__ void adjust_order(int dq) {
     if (q == 2 && dq != 1) return;
     if (dq == 1) increase_order();
     else decrease_order();
   }
 
-  __device__ void adjust_params() {
+  __device__ __forceinline__ void adjust_params() {
     if (qprime != q) {
       adjust_order(qprime - q);
       q = qprime;
@@ -332,18 +346,18 @@ struct Bdf {
     rescale();
   }
 
-  __device__ int newton_iter() {
+  __device__ __forceinline__ int newton_iter() {
     int mm = 0;
     double delp = 0.0;
     for (;;) {
-      const double r = rl1 * zn[1] + acor;
+      const double r = rl1 * zn[1][lane] + acor;
       double b = act ? gamma * ftemp - r : 0.0;
       b = getrs(b);
       if (gamrat != 1.0) b *= 2.0 / (1.0 + gamrat);
       c_nni++;
       const double del = wrms(b);
       acor += b;
-      y = zn[0] + acor;
+      y = zn[0][lane] + acor;
       if (mm > 0) crate = fmax(CRDOWN * crate, del / delp);
       const double dcon = del * fmin(1.0, crate) / tq[4];
       if (dcon <= 1.0) {
@@ -359,12 +373,12 @@ struct Bdf {
     }
   }
 
-  __device__ int nls(int nflag) {
+  __device__ __forceinline__ int nls(int nflag) {
     int convfail = (nflag == NF_FIRST || nflag == NF_PREV_ERR) ? CF_NONE : CF_OTHER;
     bool callSetup = (nflag == NF_PREV_CONV) || (nflag == NF_PREV_ERR) || (nst == 0) || (nst >= nstlp + MSBP) ||
                      (fabs(gamrat - 1.0) > DGMAX);
     for (;;) {
-      ftemp = f(zn[0]);
+      ftemp = f(zn[0][lane]);
       c_nfe++;
       if (callSetup) {
         const int ier = lsetup(convfail);
@@ -376,7 +390,7 @@ struct Bdf {
         if (ier) return NF_CONV_FAIL;
       }
       acor = 0.0;
-      y = zn[0];
+      y = zn[0][lane];
       const int ier = newton_iter();
       if (ier != NF_TRY_AGAIN) return ier;
       callSetup = true;
@@ -384,7 +398,7 @@ struct Bdf {
     }
   }
 
-  __device__ int handle_nflag(int& nflag, double saved_t, int& ncf) {
+  __device__ __forceinline__ int handle_nflag(int& nflag, double saved_t, int& ncf) {
     if (nflag == NF_OK) return 0;
     c_ncfn++;
     restore(saved_t);
@@ -397,7 +411,7 @@ struct Bdf {
     return 1;
   }
 
-  __device__ int error_test(int& nflag, double saved_t, int& nef, double& dsm) {
+  __device__ __forceinline__ int error_test(int& nflag, double saved_t, int& nef, double& dsm) {
     dsm = acnrm * tq[2];
     if (dsm <= 1.0) return 0;
     nef++;
@@ -407,7 +421,7 @@ struct Bdf {
     if (nef == MXNEF) return TGR_ST_ERR_FAIL;
     etamax = 1.0;
     if (nef <= MXNEF1) {
-      eta = 1.0 / (pow(BIAS2 * dsm, 1.0 / Lq) + ADDON);
+      eta = 1.0 / (rpow(BIAS2 * dsm, 1.0 / Lq) + ADDON);
       eta = fmax(ETAMIN, eta);
       if (nef >= SMALL_NEF) eta = fmin(eta, ETAMXF);
       rescale();
@@ -426,13 +440,13 @@ struct Bdf {
     h *= eta;
     hscale = h;
     qwait = LONG_WAIT;
-    tempv = f(zn[0]);
+    tempv = f(zn[0][lane]);
     c_nfe++;
-    zn[1] = act ? h * tempv : 0.0;
+    zn[1][lane] = act ? h * tempv : 0.0;
     return 1;
   }
 
-  __device__ void complete_step() {
+  __device__ __forceinline__ void complete_step() {
     nst++;
 #pragma unroll
     for (int i = kQmaxAbs; i >= 2; --i)
@@ -441,15 +455,15 @@ struct Bdf {
     tau[1] = h;
 #pragma unroll
     for (int j = 0; j <= kQmaxAbs; ++j)
-      if (j <= q) zn[j] += l[j] * acor;
+      if (j <= q) zn[j][lane] += l[j] * acor;
     qwait--;
     if (qwait == 1 && q != qmax) {
-      put(zn, qmax, acor);
+      zn[qmax][lane] = acor;
       saved_tq5 = tq[5];
     }
   }
 
-  __device__ void set_eta() {
+  __device__ __forceinline__ void set_eta() {
     if (eta < THRESH) {
       eta = 1.0;
       hprime = h;
@@ -459,7 +473,7 @@ struct Bdf {
     }
   }
 
-  __device__ void prepare_next(double dsm) {
+  __device__ __forceinline__ void prepare_next(double dsm) {
     if (etamax == 1.0) {
       if (qwait < 2) qwait = 2;
       qprime = q;
@@ -467,7 +481,7 @@ struct Bdf {
       eta = 1.0;
       return;
     }
-    const double etaq = 1.0 / (pow(BIAS2 * dsm, 1.0 / Lq) + ADDON);
+    const double etaq = 1.0 / (rpow(BIAS2 * dsm, 1.0 / Lq) + ADDON);
     if (qwait != 0) {
       eta = etaq;
       qprime = q;
@@ -477,15 +491,15 @@ struct Bdf {
     qwait = 2;
     double etaqm1 = 0.0;
     if (q > 1) {
-      const double ddn = wrms(pick(zn, q)) * tq[1];
-      etaqm1 = 1.0 / (pow(BIAS1 * ddn, 1.0 / q) + ADDON);
+      const double ddn = wrms(zn[q][lane]) * tq[1];
+      etaqm1 = 1.0 / (rpow(BIAS1 * ddn, 1.0 / q) + ADDON);
     }
     double etaqp1 = 0.0;
     if (q != qmax && saved_tq5 != 0.0) {
-      const double cquot = (tq[5] / saved_tq5) * pow(h / tau[2], (double)Lq);
-      const double tv = acor - cquot * pick(zn, qmax);
+      const double cquot = (tq[5] / saved_tq5) * rpow(h / tau[2], (double)Lq);
+      const double tv = acor - cquot * zn[qmax][lane];
       const double dup = wrms(tv) * tq[3];
-      etaqp1 = 1.0 / (pow(BIAS3 * dup, 1.0 / (Lq + 1)) + ADDON);
+      etaqp1 = 1.0 / (rpow(BIAS3 * dup, 1.0 / (Lq + 1)) + ADDON);
     }
     const double etam = fmax(etaqm1, fmax(etaq, etaqp1));
     if (etam < THRESH) {
@@ -500,12 +514,12 @@ struct Bdf {
     } else {
       eta = etaqp1;
       qprime = q + 1;
-      put(zn, qmax, acor);
+      zn[qmax][lane] = acor;
     }
     set_eta();
   }
 
-  __device__ int step(double tstop) {
+  __device__ __forceinline__ int step(double tstop) {
     const double saved_t = tn;
     double dsm = 0.0;
     int ncf = 0, nef = 0, nflag = NF_FIRST;
@@ -529,12 +543,12 @@ struct Bdf {
     return 0;
   }
 
-  __device__ double initial_h(double tout) {
+  __device__ __forceinline__ double initial_h(double tout) {
     const double tdist = fabs(tout - tn);
     const double tround = UROUND * fmax(fabs(tn), fabs(tout));
     const double hlb = HLB_FACTOR * tround;
-    const double den = HUB_FACTOR * fabs(zn[0]) + 1.0 / ewt;
-    const double hub_inv = uni(wave_max(act ? fabs(zn[1]) / den : 0.0));
+    const double den = HUB_FACTOR * fabs(zn[0][lane]) + 1.0 / ewt;
+    const double hub_inv = (wave_max(act ? fabs(zn[1][lane]) / den : 0.0));
     double hub = HUB_FACTOR * tdist;
     if (hub * hub_inv > 1.0) hub = 1.0 / hub_inv;
     double hg = sqrt(hlb * hub);
@@ -542,10 +556,10 @@ struct Bdf {
     double hnew = hg;
     bool hnewOK = false;
     for (int it = 1; it <= HIN_ITERS; ++it) {
-      const double yt = hg * zn[1] + zn[0];
+      const double yt = hg * zn[1][lane] + zn[0][lane];
       double fv = f(yt);
       c_nfe++;
-      fv = (fv - zn[1]) * (1.0 / hg);
+      fv = (fv - zn[1][lane]) * (1.0 / hg);
       const double yddnrm = wrms(fv);
       if (hnewOK || it == HIN_ITERS) {
         hnew = hg;
@@ -567,7 +581,7 @@ struct Bdf {
   }
 
   // integrate 0 -> tout; sol: this lane's component, updated in place
-  __device__ int integrate(double tout, double& sol) {
+  __device__ __forceinline__ int integrate(double tout, double& sol) {
     c_nfe = c_nfe_dq = c_nje = c_nsetups = c_nni = c_netf = c_ncfn = c_nlu = 0;
     nst = 0;
     const bool finite = isfinite(act ? sol : 0.0);
@@ -589,15 +603,15 @@ struct Bdf {
 #pragma unroll
     for (int j = 0; j < 6; ++j) tq[j] = 0.0;
 #pragma unroll
-    for (int j = 0; j <= kQmaxAbs; ++j) zn[j] = 0.0;
+    for (int j = 0; j <= kQmaxAbs; ++j) zn[j][lane] = 0.0;
 #pragma unroll
-    for (int c = 0; c < NEQ; ++c) J[c] = M[c] = 0.0;
+    for (int c = 0; c < NEQ; ++c) M[c] = 0.0;
     pstep = -1;
     rdiag = 0.0;
     acor = y = tempv = 0.0;
-    zn[0] = act ? sol : 0.0;
-    set_ewt(zn[0]);
-    zn[1] = f(zn[0]);
+    zn[0][lane] = act ? sol : 0.0;
+    set_ewt(zn[0][lane]);
+    zn[1][lane] = f(zn[0][lane]);
     c_nfe++;
     const double tstop = tout;
     if (!(fabs(tout - tn) >= 2.0 * UROUND * fabs(tout)) || tout <= 0.0) return TGR_ST_BAD_INPUT;
@@ -605,21 +619,21 @@ struct Bdf {
     if ((tn + h - tstop) * h > 0.0) h = (tstop - tn) * (1.0 - 4.0 * UROUND);
     hscale = h;
     hprime = h;
-    zn[1] *= h;
+    zn[1][lane] *= h;
     int nstloc = 0;
     for (;;) {
-      if (nst > 0) set_ewt(zn[0]);
+      if (nst > 0) set_ewt(zn[0][lane]);
       if (nstloc >= max_steps) {
-        sol = zn[0];
+        sol = zn[0][lane];
         return TGR_ST_TOO_MUCH_WORK;
       }
-      if (UROUND * wrms(zn[0]) > 1.0) {
-        sol = zn[0];
+      if (UROUND * wrms(zn[0][lane]) > 1.0) {
+        sol = zn[0][lane];
         return TGR_ST_TOO_MUCH_ACC;
       }
       const int k = step(tstop);
       if (k < 0) {
-        sol = zn[0];
+        sol = zn[0][lane];
         return k;
       }
       nstloc++;
@@ -632,10 +646,10 @@ struct Bdf {
       }
       if (tret >= 0.0) {
         const double s = (tret - tn) / h;
-        double v = pick(zn, q);
+        double v = zn[q][lane];
 #pragma unroll
         for (int j = kQmaxAbs - 1; j >= 0; --j)
-          if (j <= q - 1) v = zn[j] + s * v;
+          if (j <= q - 1) v = zn[j][lane] + s * v;
         sol = v;
         return TGR_ST_OK;
       }
@@ -652,17 +666,15 @@ struct Bdf {
 // ---------------------------------------------------------------------------
 template <int NEQ>
 __global__ __launch_bounds__(64) void step_kernel(StepParams p) {
-  __shared__ RhsLds lds;
-  const DevMech& m = p.m;
+  __shared__ RhsLds<NEQ> lds;
+  __shared__ double zn_lds[kQmaxAbs + 2][kWave];  // Nordsieck columns + tq, l, tau
+  const DevMech m = p.m;
   const int lane = lane_id();
   const int nsp = m.nsp;
-  int64_t cell = p.queue ? -1 : (int64_t)blockIdx.x;
   for (;;) {
-    if (p.queue) {
-      unsigned int c = 0;
-      if (lane == 0) c = atomicAdd(p.queue, 1u);
-      cell = (int64_t)(unsigned int)__builtin_amdgcn_readfirstlane((int)c);
-    }
+    unsigned int c = 0;
+    if (lane == 0) c = atomicAdd(p.queue, 1u);
+    const int64_t cell = (int64_t)(unsigned int)__builtin_amdgcn_readfirstlane((int)c);
     if (cell >= p.ncell) break;
     const int i = (int)(cell % p.nx);
     const int j = (int)((cell / p.nx) % p.ny);
@@ -676,7 +688,8 @@ __global__ __launch_bounds__(64) void step_kernel(StepParams p) {
       if (lane < nsp) sol = fmax(rho * p.sc[(int64_t)(p.isc_sc0 + lane) * p.plane + off], 0.0);
       else if (lane == nsp) sol = rho * p.sc[(int64_t)p.isc_T0 * p.plane + off];
       const double sol0 = sol;
-      Bdf<NEQ> b(m, lds, p.p_gp, p.rtol, p.atol, p.qmax, p.max_steps);
+      Bdf<NEQ> b(m, lds, zn_lds, p.ws + (int64_t)blockIdx.x * NEQ * kWave, p.p_gp, p.rtol, p.atol, p.qmax,
+                 p.max_steps);
       const int st = b.integrate(p.dt, sol);
       // unpack (finitechem.f90:1041-1042) and diagnostics (1047-1058)
       const double v = sol / rho;
@@ -684,8 +697,8 @@ __global__ __launch_bounds__(64) void step_kernel(StepParams p) {
       else if (lane == nsp) p.sc[(int64_t)p.isc_T0 * p.plane + off] = v;
       const double Tn = bcast(v, nsp);
       double h, cp;
-      thermo_lane(m, Tn, lane, h, cp);
-      const double cpm = uni(wave_sum(lane < nsp ? v * cp : 0.0));
+      thermo_lane<NEQ>(fresh(m.tab), m.tmid, Tn, lane, h, cp);
+      const double cpm = (wave_sum(lane < nsp ? v * cp : 0.0));
       if (p.hr && lane == nsp) p.hr[off] = cpm * (sol - sol0) / p.dt;
       if (p.chemsrc) {
         const double src = (sol - sol0) / p.dt;
@@ -710,32 +723,34 @@ __global__ __launch_bounds__(64) void step_kernel(StepParams p) {
         }
       }
     }
-    if (!p.queue) break;
   }
 }
 
 // ---------------------------------------------------------------------------
 // component kernels (one wave per state): production rates and RHS
 // ---------------------------------------------------------------------------
+template <int NEQ>
 __global__ __launch_bounds__(64) void prodrates_kernel(DevMech m, int64_t n, const double* C, const double* T,
                                                       const double* P, double* cdot) {
-  __shared__ RhsLds lds;
+  __shared__ RhsLds<NEQ> lds;
   const int lane = lane_id();
   const int64_t c = blockIdx.x;
   if (c >= n) return;
   const double Ck = lane < m.nsp ? C[lane * n + c] : 0.0;
-  const double v = prodrates_wave(m, lds, Ck, T[c], P[c]);
+  const double ctot = (wave_sum(Ck));
+  const double v = prodrates_wave<NEQ>(m, lds, Ck, ctot, T[c], P[c]);
   if (lane < m.nsp) cdot[lane * n + c] = v;
 }
 
+template <int NEQ>
 __global__ __launch_bounds__(64) void rhs_kernel(DevMech m, int64_t n, double p_gp, const double* y, double* f) {
-  __shared__ RhsLds lds;
+  __shared__ RhsLds<NEQ> lds;
   const int lane = lane_id();
   const int64_t c = blockIdx.x;
   if (c >= n) return;
-  const double yv = lane <= m.nsp ? y[lane * n + c] : 0.0;
-  const double v = rhs_wave(m, lds, yv, p_gp);
-  if (lane <= m.nsp) f[lane * n + c] = v;
+  const double yv = lane < NEQ ? y[lane * n + c] : 0.0;
+  const double v = rhs_wave<NEQ>(m, lds, yv, p_gp);
+  if (lane < NEQ) f[lane * n + c] = v;
 }
 
 __global__ void reset_counter_kernel(unsigned int* q) { *q = 0u; }
@@ -757,7 +772,8 @@ extern "C" int tgr_step_occupancy(int neq) {
 }
 
 extern "C" int tgr_launch_step(const StepParams* p, int neq, int grid, hipStream_t s) {
-  if (p->queue) hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, p->queue);
+  if (!p->queue || !p->ws) return -1;
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, p->queue);
   switch (neq) {
     case 10: hipLaunchKernelGGL(step_kernel<10>, dim3(grid), dim3(64), 0, s, *p); break;
     case 37: hipLaunchKernelGGL(step_kernel<37>, dim3(grid), dim3(64), 0, s, *p); break;
@@ -770,12 +786,22 @@ extern "C" int tgr_launch_step(const StepParams* p, int neq, int grid, hipStream
 extern "C" int tgr_launch_prodrates(const DevMech* m, int64_t n, const double* C, const double* T, const double* P,
                                     double* cdot, hipStream_t s) {
   if (n <= 0) return 0;
-  hipLaunchKernelGGL(prodrates_kernel, dim3((unsigned)n), dim3(64), 0, s, *m, n, C, T, P, cdot);
+  switch (m->nsp + 1) {
+    case 10: hipLaunchKernelGGL(prodrates_kernel<10>, dim3((unsigned)n), dim3(64), 0, s, *m, n, C, T, P, cdot); break;
+    case 37: hipLaunchKernelGGL(prodrates_kernel<37>, dim3((unsigned)n), dim3(64), 0, s, *m, n, C, T, P, cdot); break;
+    case 40: hipLaunchKernelGGL(prodrates_kernel<40>, dim3((unsigned)n), dim3(64), 0, s, *m, n, C, T, P, cdot); break;
+    default: return -1;
+  }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 extern "C" int tgr_launch_rhs(const DevMech* m, int64_t n, double p_gp, const double* y, double* f, hipStream_t s) {
   if (n <= 0) return 0;
-  hipLaunchKernelGGL(rhs_kernel, dim3((unsigned)n), dim3(64), 0, s, *m, n, p_gp, y, f);
+  switch (m->nsp + 1) {
+    case 10: hipLaunchKernelGGL(rhs_kernel<10>, dim3((unsigned)n), dim3(64), 0, s, *m, n, p_gp, y, f); break;
+    case 37: hipLaunchKernelGGL(rhs_kernel<37>, dim3((unsigned)n), dim3(64), 0, s, *m, n, p_gp, y, f); break;
+    case 40: hipLaunchKernelGGL(rhs_kernel<40>, dim3((unsigned)n), dim3(64), 0, s, *m, n, p_gp, y, f); break;
+    default: return -1;
+  }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }

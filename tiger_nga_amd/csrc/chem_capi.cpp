@@ -68,7 +68,6 @@ struct tgr_chem {
   int nsp = 0, neq = 0;
   double rtol = 1e-6, atol = 1e-12;
   int max_steps = 10000, qmax = 5;
-  int persistent = 1;
   int64_t grid_override = 0;
   int device = 0;
   int n_cu = 0;
@@ -77,6 +76,7 @@ struct tgr_chem {
   void* d_tables = nullptr;
   unsigned int* d_queue = nullptr;
   unsigned long long* d_totals = nullptr;
+  DevBuf b_ws;  // per-block solver workspace
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // staging for the host-array entry point
   DevBuf b_rho, b_sc, b_mask, b_hr, b_chem;
@@ -84,136 +84,120 @@ struct tgr_chem {
 
 namespace {
 
-// Builds the device tables: reactions re-ordered fall-off first (so only the
-// first 64-reaction round of a wave diverges), SoA arrays, species padded to
-// 64 lanes, and the CDOT gather schedule.
+// Builds the device table block (layout tgr::TabLayout for the mechanism's
+// compile-time shape): reactions re-ordered fall-off first (so only the first
+// 64-reaction round of a wave diverges), species padded to 64 lanes, the
+// third-body correction lists and the CDOT gather schedule.
 int build_tables(tgr_chem* c) {
   const tgr_mech_tables* t = c->t;
   const int nsp = t->nsp, nr = t->nrxn;
-  if (nsp + 1 > tgr::kMaxNeq || nr > 320 || t->n_third > 64)
-    return fail(TGR_E_MECH, "mechanism exceeds the wave-per-cell limits");
+  tgr::TabLayout L(0, 0, 0, 0, 0, 0);
+  switch (nsp + 1) {
+    case 10: L = tgr::tab_layout<10>(); break;
+    case 37: L = tgr::tab_layout<37>(); break;
+    case 40: L = tgr::tab_layout<40>(); break;
+    default: return fail(TGR_E_MECH, "no kernel instance for this mechanism size");
+  }
   std::vector<int> order;
   for (int r = 0; r < nr; ++r)
     if (t->rxn[r].kind == 2) order.push_back(r);
   const int nfo = (int)order.size();
   for (int r = 0; r < nr; ++r)
     if (t->rxn[r].kind != 2) order.push_back(r);
-  if (nfo > 255) return fail(TGR_E_MECH, "too many fall-off reactions");
-
-  std::vector<double> dbl;
-  std::vector<int> ints;
-  auto dalloc = [&](size_t n) { size_t o = dbl.size(); dbl.resize(o + n, 0.0); return o; };
-  auto ialloc = [&](size_t n) { size_t o = ints.size(); ints.resize(o + n, 0); return o; };
-
-  const size_t oW = dalloc(nsp), oiW = dalloc(nsp), oTh = dalloc(2 * 13 * 64), oEff = dalloc((size_t)nsp * 64);
-  const size_t oArr = dalloc(6 * (size_t)nr), oFo = dalloc(15 * (size_t)std::max(nfo, 1));
-  const size_t oMeta = ialloc(nr), oSp = ialloc(6 * (size_t)nr), oFoN = ialloc(std::max(nfo, 1));
-  const size_t oGs = ialloc(64), oGc = ialloc(64), oSf = ialloc(64), oSn = ialloc(64);
-
-  for (int k = 0; k < nsp; ++k) {
-    dbl[oW + k] = t->W[k];
-    dbl[oiW + k] = 1.0 / t->W[k];
+  // third-body sums as total concentration plus (eff - 1) corrections
+  std::vector<std::vector<std::pair<int, double>>> corr(t->n_third);
+  int maxe = 0;
+  for (int j = 0; j < t->n_third; ++j) {
+    for (int k = 0; k < nsp; ++k) {
+      const double e = t->eff[(size_t)j * nsp + k];
+      if (e != 1.0) corr[j].push_back({k, e - 1.0});
+    }
+    maxe = std::max(maxe, (int)corr[j].size());
   }
-  for (int rg = 0; rg < 2; ++rg)
-    for (int k = 0; k < nsp; ++k)
-      for (int q = 0; q < 13; ++q) dbl[oTh + (size_t)(rg * 13 + q) * 64 + k] = t->thermo[(size_t)(rg * nsp + k) * 13 + q];
-  for (int j = 0; j < t->n_third; ++j)
-    for (int k = 0; k < nsp; ++k) dbl[oEff + (size_t)k * 64 + j] = t->eff[(size_t)j * nsp + k];
-
+  // CDOT gather schedule: smallest chunk S with sum_k ceil(n_k/S) <= 64
   std::vector<std::vector<std::pair<int, double>>> bysp(nsp);
   for (int g = 0; g < nr; ++g) {
     const tgr_rxn& x = t->rxn[order[g]];
-    for (int q = 0; q < 3; ++q) {
-      dbl[oArr + (size_t)q * nr + g] = x.f[q];
-      dbl[oArr + (size_t)(3 + q) * nr + g] = x.b[q];
-    }
-    const int fo = (x.kind == 2) ? g : 0;
-    if (x.m + 1 > 127) return fail(TGR_E_MECH, "third-body index out of range");
-    ints[oMeta + g] = (x.kind & 3) | ((x.rev & 1) << 2) | ((x.nr & 3) << 3) | ((x.np & 3) << 5) |
-                      (((x.m + 1) & 127) << 7) | ((fo & 255) << 14);
-    for (int q = 0; q < 3; ++q) {
-      ints[oSp + (size_t)q * nr + g] = x.reac[q] < 0 ? 0 : x.reac[q];
-      ints[oSp + (size_t)(3 + q) * nr + g] = x.prod[q] < 0 ? 0 : x.prod[q];
-    }
-    if (x.kind == 2) {
-      const int nf = std::max(nfo, 1);
-      double* fo_ = &dbl[oFo];
-      for (int q = 0; q < 3; ++q) {
-        fo_[(size_t)q * nf + g] = x.f0[q];
-        fo_[(size_t)(3 + q) * nf + g] = x.b0[q];
-        fo_[(size_t)(6 + q) * nf + g] = x.fc_c[q];
-        fo_[(size_t)(9 + q) * nf + g] = x.fc_d[q];
-        fo_[(size_t)(12 + q) * nf + g] = (double)x.fc_kind[q];
-      }
-      ints[oFoN + g] = x.nfc;
-    }
-    for (int u = 0; u < x.nnu; ++u) {
-      const tgr_nu& e = t->nu[x.nu0 + u];
-      bysp[e.sp].push_back({g, e.nu});
-    }
+    for (int u = 0; u < x.nnu; ++u) bysp[t->nu[x.nu0 + u].sp].push_back({g, t->nu[x.nu0 + u].nu});
   }
-  // gather schedule: smallest chunk S with sum_k ceil(n_k/S) <= 64
   int S = 1;
   for (;; ++S) {
     int lanes = 0;
     for (int k = 0; k < nsp; ++k) lanes += ((int)bysp[k].size() + S - 1) / S;
     if (lanes <= 64) break;
   }
-  std::vector<int> gr;
-  std::vector<double> gnu;
-  int lane = 0, smax = 0;
+  if (nr != L.nrxn || nfo != L.nfo || t->n_third != L.nthird || S != L.gmax || maxe != L.maxe)
+    return fail(TGR_E_MECH, "mechanism tables do not match the compiled kernel shape");
+
+  std::vector<char> blob(L.bytes, 0);
+  auto D = [&](size_t off) -> double& { return *reinterpret_cast<double*>(&blob[off]); };
+  auto I = [&](size_t off) -> int& { return *reinterpret_cast<int*>(&blob[off]); };
+  for (int g = 0; g < nr; ++g) {
+    const tgr_rxn& x = t->rxn[order[g]];
+    const size_t rx = L.rx + (size_t)g * 64;
+    for (int q = 0; q < 3; ++q) {
+      D(rx + 8 * q) = x.f[q];
+      D(rx + 24 + 8 * q) = x.b[q];
+    }
+    if (x.m + 1 > 127 || x.m + 1 < 0) return fail(TGR_E_MECH, "third-body index out of range");
+    const int fo = (x.kind == 2) ? g : 0;
+    I(rx + 48) = (x.kind & 3) | ((x.rev & 1) << 2) | ((x.nr & 3) << 3) | ((x.np & 3) << 5) |
+                 (((x.m + 1) & 127) << 7) | ((fo & 255) << 14);
+    int rs = 0, ps = 0;
+    for (int q = 0; q < 3; ++q) {
+      rs |= (x.reac[q] < 0 ? 0 : x.reac[q]) << (8 * q);
+      ps |= (x.prod[q] < 0 ? 0 : x.prod[q]) << (8 * q);
+    }
+    I(rx + 52) = rs;
+    I(rx + 56) = ps;
+    if (x.kind == 2) {
+      const size_t f = L.fo + (size_t)g * 128;
+      int kinds = 0;
+      for (int q = 0; q < 3; ++q) {
+        D(f + 8 * q) = x.f0[q];
+        D(f + 24 + 8 * q) = x.b0[q];
+        D(f + 48 + 8 * q) = x.fc_c[q];
+        D(f + 72 + 8 * q) = x.fc_d[q];
+        kinds |= (x.fc_kind[q] & 1) << q;
+      }
+      I(f + 96) = x.nfc;
+      I(f + 100) = kinds;
+    }
+  }
+  for (int rg = 0; rg < 2; ++rg)
+    for (int k = 0; k < nsp; ++k)
+      for (int q = 0; q < 13; ++q) D(L.th + ((size_t)(rg * 13 + q) * 64 + k) * 8) = t->thermo[(size_t)(rg * nsp + k) * 13 + q];
+  for (int k = 0; k < nsp; ++k) {
+    D(L.w + 8 * k) = t->W[k];
+    D(L.iw + 8 * k) = 1.0 / t->W[k];
+  }
+  for (int j = 0; j < t->n_third; ++j) {
+    I(L.en + 4 * j) = (int)corr[j].size();
+    for (size_t e = 0; e < corr[j].size(); ++e) {
+      I(L.ek + (e * 64 + j) * 4) = corr[j][e].first;
+      D(L.ev + (e * 64 + j) * 8) = corr[j][e].second;
+    }
+  }
+  int lane = 0;
   for (int k = 0; k < nsp; ++k) {
     const int nk = (int)bysp[k].size();
     const int chunks = (nk + S - 1) / S;
-    ints[oSf + k] = lane;
-    ints[oSn + k] = chunks;
-    smax = std::max(smax, chunks);
+    I(L.sf + 4 * k) = lane;
+    I(L.sn + 4 * k) = chunks;
     for (int ch = 0; ch < chunks; ++ch, ++lane) {
-      ints[oGs + lane] = (int)gr.size();
       const int lo = ch * S, hi = std::min(nk, lo + S);
-      ints[oGc + lane] = hi - lo;
+      I(L.gc + 4 * lane) = hi - lo;
       for (int e = lo; e < hi; ++e) {
-        gr.push_back(bysp[k][e].first);
-        gnu.push_back(bysp[k][e].second);
+        I(L.gr + ((size_t)(e - lo) * 64 + lane) * 4) = bysp[k][e].first;
+        D(L.gn + ((size_t)(e - lo) * 64 + lane) * 8) = bysp[k][e].second;
       }
     }
   }
-  const size_t oGr = ialloc(std::max<size_t>(gr.size(), 1)), oGnu = dalloc(std::max<size_t>(gnu.size(), 1));
-  std::copy(gr.begin(), gr.end(), ints.begin() + oGr);
-  std::copy(gnu.begin(), gnu.end(), dbl.begin() + oGnu);
-
-  const size_t dbytes = dbl.size() * sizeof(double), ibytes = ints.size() * sizeof(int);
-  HIP_TRY(hipMalloc(&c->d_tables, dbytes + ibytes + 256));
-  char* base = (char*)c->d_tables;
-  double* dd = (double*)base;
-  int* di = (int*)(base + ((dbytes + 255) / 256) * 256);
-  HIP_TRY(hipMemcpy(dd, dbl.data(), dbytes, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(di, ints.data(), ibytes, hipMemcpyHostToDevice));
-
-  DevMech& m = c->dm;
-  m.nsp = nsp;
-  m.nrxn = nr;
-  m.nfo = std::max(nfo, 1);
-  m.nthird = t->n_third;
-  m.rounds = (nr + 63) / 64;
-  m.gmax = S;
-  m.smax = smax;
-  m.tmid = t->tmid;
-  m.W = dd + oW;
-  m.invW = dd + oiW;
-  m.thermo = dd + oTh;
-  m.effT = dd + oEff;
-  m.arr = dd + oArr;
-  m.fo = dd + oFo;
-  m.g_nu = dd + oGnu;
-  m.meta = di + oMeta;
-  m.sp = di + oSp;
-  m.fo_nfc = di + oFoN;
-  m.g_start = di + oGs;
-  m.g_cnt = di + oGc;
-  m.s_first = di + oSf;
-  m.s_n = di + oSn;
-  m.g_r = di + oGr;
+  HIP_TRY(hipMalloc(&c->d_tables, L.bytes));
+  HIP_TRY(hipMemcpy(c->d_tables, blob.data(), L.bytes, hipMemcpyHostToDevice));
+  c->dm.tab = (const char*)c->d_tables;
+  c->dm.nsp = nsp;
+  c->dm.tmid = t->tmid;
 
   tgr_mech_info& in = c->info;
   in.nsp = nsp;
@@ -225,10 +209,19 @@ int build_tables(tgr_chem* c) {
   in.n_third_body_rxn = 0;
   in.n_reversible = 0;
   in.n_fc_terms = 0;
+  in.n_conc_factors = 0;
+  in.n_rev_falloff = 0;
+  in.n_eff_corr = 0;
+  for (int j = 0; j < t->n_third; ++j) in.n_eff_corr += (int)corr[j].size();
   for (int r = 0; r < nr; ++r) {
-    in.n_third_body_rxn += t->rxn[r].kind == 1;
-    in.n_reversible += t->rxn[r].rev;
-    if (t->rxn[r].kind == 2) in.n_fc_terms += t->rxn[r].nfc;
+    const tgr_rxn& x = t->rxn[r];
+    in.n_third_body_rxn += x.kind == 1;
+    in.n_reversible += x.rev;
+    in.n_conc_factors += x.nr + (x.rev ? x.np : 0);
+    if (x.kind == 2) {
+      in.n_fc_terms += x.nfc;
+      in.n_rev_falloff += x.rev;
+    }
   }
   return 0;
 }
@@ -291,21 +284,17 @@ int run_step(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const
   p.ncell = (int64_t)nx * ny * nz;
   p.totals = c->d_totals;
   p.cell_stats = nullptr;
-  int grid;
-  if (c->persistent) {
-    p.queue = c->d_queue;
-    int per_cu = 8;
-    if (c->grid_override <= 0) {
-      int nb = tgr_step_occupancy(c->neq);
-      if (nb > 0) per_cu = nb;
-    }
-    int64_t gs = c->grid_override > 0 ? c->grid_override : (int64_t)c->n_cu * per_cu;
-    grid = (int)std::max<int64_t>(1, std::min<int64_t>(gs, p.ncell));
-  } else {
-    p.queue = nullptr;
-    if (p.ncell > 0x7fffffff) return fail(TGR_E_ARG, "too many cells for one launch");
-    grid = (int)p.ncell;
+  int per_cu = 8;
+  if (c->grid_override <= 0) {
+    int nb = tgr_step_occupancy(c->neq);
+    if (nb > 0) per_cu = nb;
   }
+  const int64_t gs = c->grid_override > 0 ? c->grid_override : (int64_t)c->n_cu * per_cu;
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(gs, p.ncell));
+  int rc0 = c->b_ws.ensure((size_t)grid * c->neq * 64 * sizeof(double));
+  if (rc0) return rc0;
+  p.queue = c->d_queue;
+  p.ws = (double*)c->b_ws.p;
   HIP_TRY(hipMemsetAsync(c->d_totals, 0, sizeof(unsigned long long) * TGR_NCOUNT, s));
   HIP_TRY(hipEventRecord(c->ev0, s));
   int rc = tgr_launch_step(&p, c->neq, grid, s);
@@ -394,6 +383,7 @@ int tgr_chem_destroy(tgr_chem* c) {
   if (c->d_totals) (void)hipFree(c->d_totals);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  c->b_ws.release();
   c->b_rho.release();
   c->b_sc.release();
   c->b_mask.release();
@@ -423,8 +413,7 @@ int tgr_chem_molar_mass(const tgr_chem* c, double* W) {
 
 int tgr_chem_set_option(tgr_chem* c, const char* key, int64_t v) {
   if (!c || !key) return fail(TGR_E_ARG, "NULL argument");
-  if (!strcmp(key, "persistent")) c->persistent = v != 0;
-  else if (!strcmp(key, "grid")) c->grid_override = v;
+  if (!strcmp(key, "grid")) c->grid_override = v;
   else if (!strcmp(key, "max_order")) {
     if (v < 1 || v > 5) return fail(TGR_E_ARG, "max_order must be 1..5");
     c->qmax = (int)v;

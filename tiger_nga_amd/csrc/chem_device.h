@@ -5,21 +5,29 @@
 // nsp holds rho*T; the remaining lanes hold 0.  The production rates of the
 // generated PRODRATES (reference combustion/HYDROGENF.f90:4-277,
 // grimech30F.f90:4-2580, DME39F.f90:4-1943) are evaluated table-driven:
-//   * lane j < n_third forms the third-body sum M_j from the concentrations
-//     broadcast through LDS;
+//   * lane j < n_third forms the third-body sum M_j = sum_k eff_jk C_k as the
+//     total concentration plus a short list of (eff_jk - 1) corrections
+//     (most efficiencies are 1), reading concentrations broadcast in LDS;
 //   * reactions are dealt to lanes round-robin (fall-off reactions first so
 //     that only the first round diverges), each lane computing k_f, k_b
 //     (Arrhenius / GETLINDRATECOEFF Troe form, HYDROGENF.f90:280-316) and
-//     the rates of progress W_f, W_b into LDS;
+//     the net rate of progress q = W_f - W_b into LDS;
 //   * CDOT is gathered deterministically: the (reaction, nu) entries of each
 //     species are split into lane-sized chunks, summed per lane, then summed
 //     per species (fixed order, no atomics).
 // The right-hand side then follows finitechem_compute_rhs
 // (combustion/finitechem.f90:93-175): rho = sum(rho*Y), T = rhoT/rho,
 // C = rho*Y/W, rhs_k = CDOT_k*W_k, rhs_T = -sum(h_k rhs_k)/cp_mix.
+//
+// Tables live in one device allocation whose layout is a compile-time
+// function of the mechanism shape (MechT), so that every table load is a
+// single base register plus lane offset plus immediate: nothing per-table has
+// to stay live in registers across the solver loop.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "fp64_math.h"
 
 namespace tgr {
 
@@ -36,7 +44,50 @@ constexpr double kTroeC1 = (double)0.67f;
 constexpr double kTroeD = (double)0.14f;
 constexpr double kInvLn10 = 0.43429448190325182765;
 
-// Reaction metadata word: kind (2 bits) | rev (1) | nr (2) | np (2) | m+1 (7) | fo (8)
+// Compile-time shape of each compiled mechanism, keyed by neq = nsp + 1.  The
+// host builds the tables for exactly this shape and refuses a mismatch.
+//   gmax: longest CDOT gather chunk per lane; maxe: longest third-body
+//   correction list.
+template <int NEQ>
+struct MechT;
+template <>
+struct MechT<10> {  // h2 (HYDROGENF)
+  static constexpr int nrxn = 25, nfo = 2, nthird = 5, gmax = 2, maxe = 3;
+};
+template <>
+struct MechT<37> {  // gri30 (grimech30F, no N chemistry)
+  static constexpr int nrxn = 219, nfo = 26, nthird = 32, gmax = 17, maxe = 8;
+};
+template <>
+struct MechT<40> {  // dme39 (DME39F)
+  static constexpr int nrxn = 175, nfo = 11, nthird = 20, gmax = 15, maxe = 6;
+};
+
+// Byte layout of the table block for a mechanism shape.
+//   RX  [rounds*64][8] doubles: A_f b_f e_f A_b b_b e_b | meta, reac, prod, 0 (int32)
+//   FO  [64][16] doubles: A0f b0f e0f A0b b0b e0b c0 c1 c2 d0 d1 d2 | nfc, kinds, 0, 0 (int32) | pad
+//   TH  [2][13][64] doubles (coefficient-major, species in lanes)
+//   W, IW [64] doubles
+//   EV  [maxe][64] doubles, EK [maxe][64] int32, EN [64] int32
+//   GN  [gmax][64] doubles, GR [gmax][64] int32, GC, SF, SN [64] int32
+struct TabLayout {
+  int nsp, nrxn, rounds, nfo, nthird, gmax, maxe;
+  size_t rx, fo, th, w, iw, ev, ek, en, gn, gr, gc, sf, sn, bytes;
+  __host__ __device__ constexpr TabLayout(int nsp_, int nrxn_, int nfo_, int nthird_, int gmax_, int maxe_)
+      : nsp(nsp_), nrxn(nrxn_), rounds((nrxn_ + 63) / 64), nfo(nfo_), nthird(nthird_), gmax(gmax_), maxe(maxe_),
+        rx(0), fo(rx + (size_t)((nrxn_ + 63) / 64) * 64 * 64), th(fo + 64 * 128), w(th + 2 * 13 * 64 * 8),
+        iw(w + 64 * 8), ev(iw + 64 * 8), ek(ev + (size_t)maxe_ * 64 * 8), en(ek + (size_t)maxe_ * 64 * 4),
+        gn(en + 64 * 4), gr(gn + (size_t)gmax_ * 64 * 8), gc(gr + (size_t)gmax_ * 64 * 4), sf(gc + 64 * 4),
+        sn(sf + 64 * 4), bytes(sn + 64 * 4) {}
+};
+
+template <int NEQ>
+__host__ __device__ constexpr TabLayout tab_layout() {
+  return TabLayout(NEQ - 1, MechT<NEQ>::nrxn, MechT<NEQ>::nfo, MechT<NEQ>::nthird, MechT<NEQ>::gmax,
+                   MechT<NEQ>::maxe);
+}
+
+// meta word: kind (2 bits) | rev (1) | nr (2) | np (2) | m+1 (7) | fo index (8)
 __host__ __device__ inline int rm_kind(int w) { return w & 3; }
 __host__ __device__ inline int rm_rev(int w) { return (w >> 2) & 1; }
 __host__ __device__ inline int rm_nr(int w) { return (w >> 3) & 3; }
@@ -44,37 +95,20 @@ __host__ __device__ inline int rm_np(int w) { return (w >> 5) & 3; }
 __host__ __device__ inline int rm_m(int w) { return ((w >> 7) & 127) - 1; }
 __host__ __device__ inline int rm_fo(int w) { return (w >> 14) & 255; }
 
-// All pointers are device pointers into one allocation owned by the context.
+// Device view of a mechanism: the table block plus a few sizes.
 struct DevMech {
-  int nsp, nrxn, nfo, nthird;
-  int rounds;      // ceil(nrxn / 64)
-  int gmax;        // longest CDOT gather chunk
-  int smax;        // most chunks of one species
+  const char* tab;
+  int nsp;
   double tmid;
-  const double* W;       // [nsp]
-  const double* invW;    // [nsp]
-  const double* thermo;  // [2][13][64]: coefficient-major, species in lanes
-  const double* effT;    // [nsp][64]: efficiency of species k in sum j (lane j)
-  const double* arr;     // [6][nrxn]: A_f b_f e_f A_b b_b e_b (fall-off: k_inf)
-  const int* meta;       // [nrxn]
-  const int* sp;         // [6][nrxn]: reac0..2 prod0..2
-  const double* fo;      // [15][nfo]: A0f b0f e0f A0b b0b e0b c0 c1 c2 d0 d1 d2 kind0 kind1 kind2
-  const int* fo_nfc;     // [nfo]
-  const int* g_start;    // [64] chunk start per lane
-  const int* g_cnt;      // [64] chunk length per lane (0 idle)
-  const int* g_r;        // [nnz] reaction of entry
-  const double* g_nu;    // [nnz] coefficient of entry
-  const int* s_first;    // [64] first chunk lane of species k (lane k)
-  const int* s_n;        // [64] number of chunks of species k
 };
 
 // Per-wave LDS scratch for one right-hand-side evaluation.
+template <int NEQ>
 struct RhsLds {
   double C[kWave];
   double M[kWave];
   double part[kWave];
-  double Wf[256 + 64];
-  double Wb[256 + 64];
+  double q[((MechT<NEQ>::nrxn + 63) / 64) * 64];
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -86,16 +120,12 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
-  return v;
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
-  return v;
+// DPP move of a double (both halves), lanes without a source get `old`.
+template <int CTRL, int ROWMASK, int BANKMASK, bool BOUND0>
+__device__ __forceinline__ double dpp(double old, double v) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(v), CTRL, ROWMASK, BANKMASK, BOUND0);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(v), CTRL, ROWMASK, BANKMASK, BOUND0);
+  return __hiloint2double(hi, lo);
 }
 
 // value of v in lane `src` (src wave-uniform), as a wave-uniform scalar
@@ -105,124 +135,165 @@ __device__ __forceinline__ double bcast(double v, int src) {
   return __hiloint2double(hi, lo);
 }
 
-// a value that is equal in all lanes, made visibly uniform to the compiler
-__device__ __forceinline__ double uni(double v) {
-  int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
-  int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
-  return __hiloint2double(hi, lo);
+// Sum over the 64 lanes with the row_shr / row_bcast DPP tree (total lands
+// in lane 63), returned wave-uniform.  Fixed order: deterministic.
+__device__ __forceinline__ double wave_sum(double v) {
+  double s = v + dpp<0x111, 0xf, 0xf, true>(0.0, v);  // row_shr:1
+  s = s + dpp<0x112, 0xf, 0xf, true>(0.0, v);         // row_shr:2
+  s = s + dpp<0x113, 0xf, 0xf, true>(0.0, v);         // row_shr:3
+  s = s + dpp<0x114, 0xf, 0xe, false>(0.0, s);        // row_shr:4, banks 1-3
+  s = s + dpp<0x118, 0xf, 0xc, false>(0.0, s);        // row_shr:8, banks 2-3
+  s = s + dpp<0x142, 0xa, 0xf, false>(0.0, s);        // row_bcast:15, rows 1,3
+  s = s + dpp<0x143, 0xc, 0xf, false>(0.0, s);        // row_bcast:31, rows 2,3
+  return bcast(s, 63);
+}
+
+// Maximum over the 64 lanes (same tree; lanes without a source keep their value).
+__device__ __forceinline__ double wave_max(double v) {
+  double s = fmax(v, dpp<0x111, 0xf, 0xf, false>(v, v));
+  s = fmax(s, dpp<0x112, 0xf, 0xf, false>(v, v));
+  s = fmax(s, dpp<0x113, 0xf, 0xf, false>(v, v));
+  s = fmax(s, dpp<0x114, 0xf, 0xe, false>(s, s));
+  s = fmax(s, dpp<0x118, 0xf, 0xc, false>(s, s));
+  s = fmax(s, dpp<0x142, 0xa, 0xf, false>(s, s));
+  s = fmax(s, dpp<0x143, 0xc, 0xf, false>(s, s));
+  return bcast(s, 63);
+}
+
+// Returns the table base through an empty asm statement so that table loads
+// are issued where they are used: without it the compiler hoists every
+// loop-invariant table load out of the solver loops and keeps hundreds of
+// values live in registers across the whole integration.
+template <class T>
+__device__ __forceinline__ T* fresh(T* p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
+template <class T>
+__device__ __forceinline__ T ld(const char* base, size_t off) {
+  return *reinterpret_cast<const T*>(base + off);
 }
 
 __device__ __forceinline__ double arrhenius(double A, double b, double e, double lt, double irt) {
-  return A * exp(b * lt + e * irt);
+  return A * fexp(fma(b, lt, e * irt));
 }
 
 // NASA-7 enthalpy / heat capacity of the species in this lane
 // (COMPTHERMODATA, HYDROGENF.f90:514-673); lanes >= nsp return 0.
-__device__ __forceinline__ void thermo_lane(const DevMech& m, double T, int lane, double& h, double& cp) {
+template <int NEQ>
+__device__ __forceinline__ void thermo_lane(const char* tab, double tmid, double T, int lane, double& h, double& cp) {
+  constexpr TabLayout L = tab_layout<NEQ>();
   h = 0.0;
   cp = 0.0;
-  if (lane < m.nsp) {
-    const double* c = m.thermo + (T > m.tmid ? 0 : 13 * kWave) + lane;
-    double a = c[0], c1 = c[kWave], c2 = c[2 * kWave], c3 = c[3 * kWave], c4 = c[4 * kWave], c5 = c[5 * kWave],
-           c6 = c[6 * kWave];
-    double ac = c[7 * kWave], d1 = c[8 * kWave], d2 = c[9 * kWave], d3 = c[10 * kWave], d4 = c[11 * kWave],
-           d5 = c[12 * kWave];
+  if (lane < NEQ - 1) {
+    const char* c = tab + L.th + (T > tmid ? 0 : 13 * 64 * 8) + lane * 8;
+    const double a = ld<double>(c, 0), c1 = ld<double>(c, 512), c2 = ld<double>(c, 1024),
+                 c3 = ld<double>(c, 1536), c4 = ld<double>(c, 2048), c5 = ld<double>(c, 2560),
+                 c6 = ld<double>(c, 3072);
+    const double ac = ld<double>(c, 3584), d1 = ld<double>(c, 4096), d2 = ld<double>(c, 4608),
+                 d3 = ld<double>(c, 5120), d4 = ld<double>(c, 5632), d5 = ld<double>(c, 6144);
     h = a * (T * (c1 + T * (c2 + T * (c3 + T * (c4 + T * c5)))) + c6);
     cp = ac * (d1 + T * (d2 + T * (d3 + T * (d4 + T * d5))));
   }
 }
 
+// Troe fall-off rate (GETLINDRATECOEFF, HYDROGENF.f90:280-316, ITROE = 1):
+// k = k_inf * F * Pr / (1 + Pr), Pr = k0 * conc / max(k_inf, 1e-60),
+// log10 F = log10 Fc / (1 + ((log10 Pr + c) / (n - d (log10 Pr + c)))^2).
+__device__ __forceinline__ double troe(double kinf, double k0, double conc, double lnfc, double ntmp, double cco) {
+  const double pr = k0 * conc / fmax(kinf, 1.0e-60);
+  const double lg = fma(flog(pr), kInvLn10, cco);
+  const double x = lg / fma(-kTroeD, lg, ntmp);
+  const double F = fexp(lnfc / fma(x, x, 1.0));
+  return kinf * F * pr / (1.0 + pr);
+}
+
 // Production rates CDOT (kmol/m3/s) of this lane's species from the
-// concentrations (lane k holds C_k), temperature and pressure.
-__device__ inline double prodrates_wave(const DevMech& m, RhsLds& L, double Ck, double T, double P) {
+// concentrations (lane k holds C_k), temperature and pressure.  ctot is the
+// total concentration sum_k C_k (wave-uniform).
+template <int NEQ>
+__device__ __forceinline__ double prodrates_wave(const DevMech& m, RhsLds<NEQ>& S, double Ck, double ctot, double T,
+                                                 double P) {
+  using MT = MechT<NEQ>;
+  constexpr TabLayout L = tab_layout<NEQ>();
   const int lane = lane_id();
-  const double lt = log(T);
+  const char* tab = fresh(m.tab);
+  const double lt = flog(T);
   const double rt = kRgasF4 * T;
   const double irt = 1.0 / rt;
-  L.C[lane] = (lane < m.nsp) ? Ck : 0.0;
+  S.C[lane] = Ck;
   wave_sync();
-  // third-body sums
-  if (lane < m.nthird) {
-    double s = 0.0;
-    const double* e = m.effT + lane;
-    for (int k = 0; k < m.nsp; ++k) s += e[k * kWave] * L.C[k];
-    L.M[lane] = s;
+  // third-body sums: M_j = ctot + sum_e (eff_je - 1) C_e
+  if (lane < MT::nthird) {
+    double s = ctot;
+    const int ne = ld<int>(tab, L.en + lane * 4);
+#pragma unroll
+    for (int e = 0; e < MT::maxe; ++e)
+      if (e < ne) s = fma(ld<double>(tab, L.ev + (e * 64 + lane) * 8), S.C[ld<int>(tab, L.ek + (e * 64 + lane) * 4)], s);
+    S.M[lane] = s;
   }
   wave_sync();
-  // rates of progress, one reaction per lane per round
-  for (int rd = 0; rd < m.rounds; ++rd) {
-    const int r = rd * kWave + lane;
-    if (r < m.nrxn) {
-      const int w = m.meta[r];
+  // rates of progress, one reaction per lane per round (fall-off in round 0)
+#pragma unroll
+  for (int rd = 0; rd < L.rounds; ++rd) {
+    const int r = rd * 64 + lane;
+    if (r < MT::nrxn) {
+      const char* rx = tab + L.rx + (size_t)r * 64;
+      const int w = ld<int>(rx, 48), rs = ld<int>(rx, 52), ps = ld<int>(rx, 56);
       const int kind = rm_kind(w), rev = rm_rev(w), nr = rm_nr(w), np = rm_np(w), mi = rm_m(w);
-      const int n = m.nrxn;
-      double kf = arrhenius(m.arr[r], m.arr[n + r], m.arr[2 * n + r], lt, irt);
-      double kb = rev ? arrhenius(m.arr[3 * n + r], m.arr[4 * n + r], m.arr[5 * n + r], lt, irt) : 0.0;
-      if (kind == 2) {
-        // Troe fall-off, GETLINDRATECOEFF (HYDROGENF.f90:280-316)
-        const int f = rm_fo(w), nf = m.nfo;
-        const double* fo = m.fo + f;
-        const int nfc = m.fo_nfc[f];
+      double kf = arrhenius(ld<double>(rx, 0), ld<double>(rx, 8), ld<double>(rx, 16), lt, irt);
+      double kb = rev ? arrhenius(ld<double>(rx, 24), ld<double>(rx, 32), ld<double>(rx, 40), lt, irt) : 0.0;
+      if (rd == 0 && kind == 2) {
+        const char* fo = tab + L.fo + (size_t)rm_fo(w) * 128;
+        const int nfc = ld<int>(fo, 96), kinds = ld<int>(fo, 100);
         double fc = 0.0;
-        for (int i = 0; i < nfc; ++i) {
-          double c = fo[(6 + i) * nf], d = fo[(9 + i) * nf];
-          double x = (fo[(12 + i) * nf] == 0.0) ? -(T / d) : -(d / T);
-          fc = fc + c * exp(x);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          if (i < nfc) {
+            const double c = ld<double>(fo, 48 + 8 * i), d = ld<double>(fo, 72 + 8 * i);
+            const double x = ((kinds >> i) & 1) ? -(d / T) : -(T / d);
+            fc = fma(c, fexp(x), fc);
+          }
         }
-        const double conc = (mi >= 0 && L.M[mi] > 0.0) ? L.M[mi] : P / (kRgasF4 * T);
-        const double lnfc = log(fc);
+        const double conc = (mi >= 0 && S.M[mi] > 0.0) ? S.M[mi] : P / rt;
+        const double lnfc = flog(fc);
         const double lfc = lnfc * kInvLn10;
-        const double ntmp = kTroeN0 - kTroeN1 * lfc;
-        const double cco = -kTroeC0 - kTroeC1 * lfc;
-        {
-          double k0 = arrhenius(fo[0], fo[nf], fo[2 * nf], lt, irt);
-          double pr = k0 * conc / fmax(kf, 1.0e-60);
-          double lg = log(pr) * kInvLn10 + cco;
-          double x = lg / (ntmp - kTroeD * lg);
-          double F = exp(lnfc / (x * x + 1.0));
-          kf = kf * F * pr / (1.0 + pr);
-        }
-        if (rev) {
-          double k0 = arrhenius(fo[3 * nf], fo[4 * nf], fo[5 * nf], lt, irt);
-          double pr = k0 * conc / fmax(kb, 1.0e-60);
-          double lg = log(pr) * kInvLn10 + cco;
-          double x = lg / (ntmp - kTroeD * lg);
-          double F = exp(lnfc / (x * x + 1.0));
-          kb = kb * F * pr / (1.0 + pr);
-        }
+        const double ntmp = fma(-kTroeN1, lfc, kTroeN0);
+        const double cco = fma(-kTroeC1, lfc, -kTroeC0);
+        kf = troe(kf, arrhenius(ld<double>(fo, 0), ld<double>(fo, 8), ld<double>(fo, 16), lt, irt), conc, lnfc,
+                  ntmp, cco);
+        if (rev)
+          kb = troe(kb, arrhenius(ld<double>(fo, 24), ld<double>(fo, 32), ld<double>(fo, 40), lt, irt), conc, lnfc,
+                    ntmp, cco);
       }
       double wf = kf, wb = kb;
-      const int* s = m.sp + r;
-      if (nr > 0) wf *= L.C[s[0]];
-      if (nr > 1) wf *= L.C[s[n]];
-      if (nr > 2) wf *= L.C[s[2 * n]];
-      if (np > 0) wb *= L.C[s[3 * n]];
-      if (np > 1) wb *= L.C[s[4 * n]];
-      if (np > 2) wb *= L.C[s[5 * n]];
-      if (kind == 1) {
-        wf *= L.M[mi];
-        wb *= L.M[mi];
-      }
-      L.Wf[r] = wf;
-      L.Wb[r] = wb;
+      if (nr > 0) wf *= S.C[rs & 255];
+      if (nr > 1) wf *= S.C[(rs >> 8) & 255];
+      if (nr > 2) wf *= S.C[(rs >> 16) & 255];
+      if (np > 0) wb *= S.C[ps & 255];
+      if (np > 1) wb *= S.C[(ps >> 8) & 255];
+      if (np > 2) wb *= S.C[(ps >> 16) & 255];
+      double q = wf - wb;
+      if (kind == 1) q *= S.M[mi];
+      S.q[r] = q;
     }
   }
   wave_sync();
   // deterministic gather of CDOT
   {
     double acc = 0.0;
-    const int st = m.g_start[lane], cnt = m.g_cnt[lane];
-    for (int q = 0; q < cnt; ++q) {
-      const int r = m.g_r[st + q];
-      acc += m.g_nu[st + q] * (L.Wf[r] - L.Wb[r]);
-    }
-    L.part[lane] = acc;
+    const int cnt = ld<int>(tab, L.gc + lane * 4);
+#pragma unroll
+    for (int i = 0; i < MT::gmax; ++i)
+      if (i < cnt) acc = fma(ld<double>(tab, L.gn + (i * 64 + lane) * 8), S.q[ld<int>(tab, L.gr + (i * 64 + lane) * 4)], acc);
+    S.part[lane] = acc;
   }
   wave_sync();
   double cdot = 0.0;
-  if (lane < m.nsp) {
-    const int f0 = m.s_first[lane], nn = m.s_n[lane];
-    for (int q = 0; q < nn; ++q) cdot += L.part[f0 + q];
+  if (lane < NEQ - 1) {
+    const int f0 = ld<int>(tab, L.sf + lane * 4), nn = ld<int>(tab, L.sn + lane * 4);
+    for (int i = 0; i < nn; ++i) cdot += S.part[f0 + i];
   }
   wave_sync();
   return cdot;
@@ -231,19 +302,24 @@ __device__ inline double prodrates_wave(const DevMech& m, RhsLds& L, double Ck, 
 // finitechem_compute_rhs (finitechem.f90:93-175), constant-pressure branch.
 // y: this lane's component of [rho*Y (nsp), rho*T]; returns the same lane of
 // the right-hand side.
-__device__ inline double rhs_wave(const DevMech& m, RhsLds& L, double y, double p_gp) {
+template <int NEQ>
+__device__ __forceinline__ double rhs_wave(const DevMech& m, RhsLds<NEQ>& S, double y, double p_gp) {
+  constexpr int nsp = NEQ - 1;
+  constexpr TabLayout L = tab_layout<NEQ>();
   const int lane = lane_id();
-  const int nsp = m.nsp;
-  const double rho = uni(wave_sum(lane < nsp ? y : 0.0));
+  const bool sp = lane < nsp;
+  const char* tab = fresh(m.tab);
+  const double rho = wave_sum(sp ? y : 0.0);
   const double T = bcast(y, nsp) / rho;
   double h, cp;
-  thermo_lane(m, T, lane, h, cp);
-  const double cpm = uni(wave_sum(lane < nsp ? (y / rho) * cp : 0.0));
-  const double Ck = (lane < nsp) ? y * m.invW[lane < nsp ? lane : 0] : 0.0;
-  const double cdot = prodrates_wave(m, L, Ck, T, p_gp);
-  const double rk = (lane < nsp) ? cdot * m.W[lane < nsp ? lane : 0] : 0.0;
-  const double tmp = uni(wave_sum(-h * rk));
-  return (lane < nsp) ? rk : (lane == nsp ? tmp / cpm : 0.0);
+  thermo_lane<NEQ>(tab, m.tmid, T, lane, h, cp);
+  const double Ck = sp ? y * ld<double>(tab, L.iw + lane * 8) : 0.0;
+  const double cpm = wave_sum(sp ? (y / rho) * cp : 0.0);
+  const double ctot = wave_sum(Ck);
+  const double cdot = prodrates_wave<NEQ>(m, S, Ck, ctot, T, p_gp);
+  const double rk = sp ? cdot * ld<double>(tab, L.w + lane * 8) : 0.0;
+  const double tmp = wave_sum(-h * rk);
+  return sp ? rk : (lane == nsp ? tmp / cpm : 0.0);
 }
 
 }  // namespace tgr

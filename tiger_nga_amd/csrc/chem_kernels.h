@@ -39,7 +39,8 @@ struct StepParams {
   double* chemsrc;  // chemsrc_gp(..., 3) or null
   int idx_h2o, idx_co, idx_co2;
   int64_t ncell;
-  unsigned int* queue;           // device work counter (persistent grid) or null (one cell per block)
+  unsigned int* queue;           // device work counter
+  double* ws;                    // per-block workspace: saved Jacobian, neq * 64 doubles per block
   int* cell_stats;               // [ncell][TGR_NCOUNT] or null
   unsigned long long* totals;    // [TGR_NCOUNT] (last: failed cells) or null
 };

@@ -41,7 +41,8 @@ class _Stats(ctypes.Structure):
 
 class _Info(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
-        "nsp", "nrxn", "nslot", "n_third", "n_falloff", "n_third_body_rxn", "n_reversible", "n_nu", "n_fc_terms")]
+        "nsp", "nrxn", "nslot", "n_third", "n_falloff", "n_third_body_rxn", "n_reversible", "n_nu", "n_fc_terms",
+        "n_conc_factors", "n_eff_corr", "n_rev_falloff")]
 
 
 @dataclass
