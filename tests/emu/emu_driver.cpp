@@ -1,0 +1,136 @@
+// CPU build of the wave-per-cell chemistry solver (test infrastructure).
+//
+// Runs tgr::process_cell -- the same source the HIP step kernel executes --
+// with 64 fibers standing in for the lanes of one wavefront, over a
+// packed batch of cells (SoA: sc[i * ncell + c], species then rho*T / T as in
+// the Fortran block with one cell row).  Used by tests/test_wave_emu.py to
+// check the GPU solver logic against the C oracle without a GPU, and with
+// sanitizers for bounds checks of the device code.
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "bdf_wave.h"
+#include "mech_dme39.h"
+#include "mech_gri30.h"
+#include "mech_h2.h"
+#include "tables_host.h"
+
+namespace {
+
+const tgr_mech_tables* find_mech(const char* name) {
+  if (!std::strcmp(name, "h2")) return &h2_tables;
+  if (!std::strcmp(name, "gri30")) return &gri30_tables;
+  if (!std::strcmp(name, "dme39")) return &dme39_tables;
+  return nullptr;
+}
+
+int species_index(const tgr_mech_tables* t, const char* name) {
+  for (int k = 0; k < t->nsp; ++k)
+    if (!std::strcmp(t->species[k], name)) return k;
+  return -1;
+}
+
+template <int NEQ>
+struct Run {
+  const tgr::StepParams* p;
+  int* counters;
+  tgr::RhsLds<NEQ> lds;
+  // per-lane copy of the LDS history block: every lane reads and writes only
+  // its own column, plus one wave-uniform row that all lanes update with the
+  // same values -- one update in SIMT, but sequential fibers sharing it would
+  // apply it 64 times
+  double zn[64][tgr::kQmaxAbs + 2][tgr::kWave];
+  std::vector<double> ws;
+  tgr_emu::Wave wave;
+  int finished = 0;
+
+  static void lane_main(unsigned hi, unsigned lo, int l) {
+    Run* r = (Run*)(((uintptr_t)hi << 32) | (uintptr_t)lo);
+    for (int64_t c = 0; c < r->p->ncell; ++c) {
+      int cnt[TGR_NCOUNT] = {0};
+      const int done = tgr::process_cell<NEQ>(*r->p, r->p->m, c, r->lds, r->zn[l], r->ws.data(), cnt);
+      if (l == 0 && r->counters)
+        for (int q = 0; q < TGR_NCOUNT; ++q) r->counters[c * TGR_NCOUNT + q] = done ? cnt[q] : 0;
+    }
+    // lanes leave in order; the last one returns to the caller
+    r->finished++;
+    if (r->finished == 64) {
+      setcontext(&r->wave.home);
+    } else {
+      const int nx = (l + 1) & 63;
+      tgr_emu::tl_lane = nx;
+      setcontext(&r->wave.ctx[nx]);
+    }
+  }
+};
+
+template <int NEQ>
+void run(const tgr::StepParams& p, int* counters) {
+  Run<NEQ>* r = new Run<NEQ>();
+  r->p = &p;
+  r->counters = counters;
+  r->ws.assign((size_t)NEQ * tgr::kWave, 0.0);
+  const size_t stack = 1 << 20;
+  std::vector<char> stacks(stack * 64);
+  tgr_emu::tw = &r->wave;
+  for (int l = 0; l < 64; ++l) {
+    getcontext(&r->wave.ctx[l]);
+    r->wave.ctx[l].uc_stack.ss_sp = stacks.data() + stack * l;
+    r->wave.ctx[l].uc_stack.ss_size = stack;
+    r->wave.ctx[l].uc_link = nullptr;
+    const uintptr_t a = (uintptr_t)r;
+    makecontext(&r->wave.ctx[l], (void (*)())Run<NEQ>::lane_main, 3, (unsigned)(a >> 32), (unsigned)(a & 0xffffffffu), l);
+  }
+  tgr_emu::tl_lane = 0;
+  swapcontext(&r->wave.home, &r->wave.ctx[0]);
+  delete r;
+}
+
+}  // namespace
+
+// One chemistry step over ncell packed cells: rho[ncell], sc[(nsp+1) * ncell]
+// (mass fractions then T, overwritten), hr[ncell], chemsrc[3 * ncell],
+// counters[ncell * 10] (nst nfe nfe_dq nje nsetups nni netf ncfn nlu status).
+extern "C" int emu_step(const char* mech, long ncell, double* rho, double* sc, double dt, double p_gp, double rtol,
+                        double atol, int max_steps, int* counters, double* hr, double* chemsrc) {
+  const tgr_mech_tables* t = find_mech(mech);
+  if (!t) return -1;
+  std::vector<char> blob;
+  tgr_mech_info info;
+  std::string err;
+  if (tgr::build_table_blob(t, blob, &info, &err)) return -2;
+  tgr::StepParams p{};
+  p.m.tab = blob.data();
+  p.m.nsp = t->nsp;
+  p.m.tmid = t->tmid;
+  p.dt = dt;
+  p.p_gp = p_gp;
+  p.rtol = rtol;
+  p.atol = atol;
+  p.max_steps = max_steps;
+  p.qmax = 5;
+  p.nx = (int)ncell;
+  p.ny = p.nz = 1;
+  p.sx = 1;
+  p.sy = p.sz = ncell;
+  p.base = 0;
+  p.plane = ncell;
+  p.isc_sc0 = 0;
+  p.isc_T0 = t->nsp;
+  p.rho = rho;
+  p.sc = sc;
+  p.hr = hr;
+  p.chemsrc = chemsrc;
+  p.idx_h2o = species_index(t, "H2O");
+  p.idx_co = species_index(t, "CO");
+  p.idx_co2 = species_index(t, "CO2");
+  p.ncell = ncell;
+  switch (t->nsp + 1) {
+    case 10: run<10>(p, counters); break;
+    case 37: run<37>(p, counters); break;
+    case 40: run<40>(p, counters); break;
+    default: return -3;
+  }
+  return 0;
+}

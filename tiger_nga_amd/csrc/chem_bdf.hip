@@ -1,726 +1,39 @@
-// Plain FP64 chemistry step on CDNA4: one wavefront integrates one cell.
-//
-// Replaces the body of finitechem_source_chemistry
-// (reference combustion/finitechem.f90:962-1071) and the SUNDIALS CVODE 2.5.0
-// solver it drives (BDF, Newton, dense difference-quotient Jacobian, scalar
-// tolerances, tstop = tout = dt, MAX_NSTEPS; finitechem.f90:1014-1034).  The
-// integrator follows the published CVODE algorithm step for step (the same
-// restatement the CPU oracle holds), laid out for a 64-wide wavefront:
-//   * vectors of length neq = nsp+1 live one component per lane;
-//   * the iteration matrix M = I - gamma*J lives one ROW per lane in
-//     registers (NEQ doubles); the saved Jacobian J (reused for up to 50
-//     steps) lives in a per-wave HBM workspace, column-major with the rows
-//     in lanes, so each column store / row load is one coalesced access;
-//   * LU with partial pivoting keeps rows in their lanes (implicit row
-//     exchange): the pivot is a wave arg-max, the pivot row is broadcast with
-//     v_readlane, and each lane records the elimination step of its row;
-//   * all step-size / order / Newton control is wave-uniform scalar code.
-// Cells are pulled from a device work counter (persistent grid) so that cells
-// whose step counts differ by 10-100x keep every wave busy.
+// HIP kernels of the chemistry step (gfx950): the persistent step kernel,
+// the component kernels used by the parity tests, and their launchers.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "chem_device.h"
+#include "bdf_wave.h"
 #include "chem_kernels.h"
 
 namespace tgr {
 
-// CVODE defaults and algorithm constants (same values as the CPU oracle)
-constexpr int kQmaxAbs = 5;
-constexpr int MXNCF = 10, MXNEF = 7, MXNEF1 = 3, SMALL_NEF = 2, LONG_WAIT = 10, NLS_MAXCOR = 3;
-constexpr int MSBP = 20, MSBJ = 50, SMALL_NST = 10;
-constexpr double CRDOWN = 0.3, DGMAX = 0.3, RDIV = 2.0, DGMAX_J = 0.2;
-constexpr double ETAMX1 = 10000.0, ETAMX2 = 10.0, ETAMX3 = 10.0, ETAMXF = 0.2, ETAMIN = 0.1, ETACF = 0.25;
-constexpr double ADDON = 1.0e-6, BIAS1 = 6.0, BIAS2 = 6.0, BIAS3 = 10.0, THRESH = 1.5, NLSCOEF = 0.1;
-constexpr double HLB_FACTOR = 100.0, HUB_FACTOR = 0.1, H_BIAS = 0.5, FUZZ_FACTOR = 100.0, MIN_INC_MULT = 1000.0;
-constexpr int HIN_ITERS = 4;
-constexpr double UROUND = 2.2204460492503131e-16;
-
-enum { NF_FIRST = 0, NF_PREV_CONV = 1, NF_PREV_ERR = 2, NF_OK = 3, NF_CONV_FAIL = 4, NF_TRY_AGAIN = 5 };
-enum { CF_NONE = 0, CF_BAD_J = 1, CF_OTHER = 2 };
-
-// x^y for x >= 0 through the project's exp/log (step-size ratios only)
-__device__ __forceinline__ double rpow(double x, double y) { return fexp(y * flog(x)); }
-
-template <class T, int N>
-__device__ __forceinline__ T pick(const T (&a)[N], int i) {
-  T r = a[0];
-#pragma unroll
-  for (int k = 1; k < N; ++k) r = (i == k) ? a[k] : r;
-  return r;
-}
-
-template <class T, int N>
-__device__ __forceinline__ void put(T (&a)[N], int i, T v) {
-#pragma unroll
-  for (int k = 0; k < N; ++k) a[k] = (i == k) ? v : a[k];
-}
-
-template <int NEQ>
-struct Bdf {
-  const DevMech m;  // by value: fields stay in scalar registers
-  RhsLds<NEQ>& L;
-  double p_gp, rtol, atol;
-  int lane, n, qmax, max_steps;
-  bool act;
-  // per-lane vectors; the Nordsieck history lives in LDS (lane-private
-  // columns), so that zn[q] with a run-time order q is a plain address
-  double (&zn)[kQmaxAbs + 2][kWave];
-  double ewt, acor, y, tempv, ftemp;
-  double* jw;  // saved Jacobian: jw[c * 64 + lane] = J(lane, c)
-  double M[NEQ];
-  int pstep;
-  double rdiag;
-  // wave-uniform control state
-  double tn, h, hprime, hscale, eta, etamax, rl1, gamma, gammap, gamrat, crate, acnrm, saved_tq5;
-  // small wave-uniform arrays indexed by the run-time order (LDS, every lane
-  // writes the same value)
-  double* tq;
-  double* l;
-  double* tau;
-  int q, qprime, qwait, Lq, jcur, nst, nstlp, nstlj;
-  int c_nfe, c_nfe_dq, c_nje, c_nsetups, c_nni, c_netf, c_ncfn, c_nlu;
-
-  __device__ Bdf(const DevMech& mech, RhsLds<NEQ>& lds, double (&znl)[kQmaxAbs + 2][kWave], double* ws, double pgp,
-                 double rt, double at, int qm, int ms)
-      : m(mech), L(lds), p_gp(pgp), rtol(rt), atol(at), max_steps(ms), zn(znl), jw(ws) {
-    tq = &znl[kQmaxAbs + 1][0];
-    l = tq + 8;
-    tau = tq + 16;
-    lane = lane_id();
-    n = NEQ;
-    act = lane < NEQ;
-    qmax = (qm >= 1 && qm <= kQmaxAbs) ? qm : kQmaxAbs;
-  }
-
-  __device__ __forceinline__ double f(double yv) { return rhs_wave<NEQ>(m, L, act ? yv : 0.0, p_gp); }
-
-  __device__ __forceinline__ double wrms(double x) {
-    double v = act ? x * ewt : 0.0;
-    return sqrt((wave_sum(v * v)) / n);
-  }
-
-  __device__ __forceinline__ void set_ewt(double yv) { ewt = act ? 1.0 / (rtol * fabs(yv) + atol) : 0.0; }
-
-  // ---- dense linear algebra, one matrix row per lane --------------------
-  __device__ __forceinline__ int getrf() {
-    pstep = act ? NEQ : -1;
-#pragma unroll
-    for (int k = 0; k < NEQ; ++k) {
-      const bool cand = (pstep == NEQ);
-      const double a = cand ? fabs(M[k]) : -1.0;
-      const double amax = (wave_max(a));
-      if (!(amax > 0.0)) return k + 1;
-      const uint64_t bal = __ballot(cand && a == amax);
-      const int p = __builtin_amdgcn_readfirstlane(bal ? (int)__builtin_ctzll(bal) : 0);
-      const double piv = bcast(M[k], p);
-      const double rp = 1.0 / piv;
-      if (lane == p) {
-        pstep = k;
-        rdiag = rp;
-      }
-      const bool elim = cand && lane != p;
-      const double lf = M[k] * rp;
-      if (elim) M[k] = lf;
-#pragma unroll
-      for (int c = k + 1; c < NEQ; ++c) {
-        const double u = bcast(M[c], p);
-        if (elim) M[c] -= u * lf;
-      }
-    }
-    return 0;
-  }
-
-  // solve M x = b with the factors from getrf; b, x: one component per lane
-  __device__ __forceinline__ double getrs(double b) {
-    double xo = 0.0;
-#pragma unroll
-    for (int k = 0; k < NEQ; ++k) {
-      const uint64_t bal = __ballot(pstep == k);
-      const int p = __builtin_amdgcn_readfirstlane(bal ? (int)__builtin_ctzll(bal) : 0);
-      const double yk = bcast(b, p);
-      if (pstep > k) b -= M[k] * yk;
-    }
-#pragma unroll
-    for (int k = NEQ - 1; k >= 0; --k) {
-      const uint64_t bal = __ballot(pstep == k);
-      const int p = __builtin_amdgcn_readfirstlane(bal ? (int)__builtin_ctzll(bal) : 0);
-      const double xk = bcast(b, p) * bcast(rdiag, p);
-      if (pstep >= 0 && pstep < k) b -= M[k] * xk;
-      xo = (lane == k) ? xk : xo;
-    }
-    return xo;
-  }
-
-  // difference-quotient Jacobian, column j perturbs component j
-  __device__ __forceinline__ void dq_jac(double yv, double fy) {
-    const double srur = sqrt(UROUND);
-    const double fnorm = wrms(fy);
-    const double minInc = (fnorm != 0.0) ? (MIN_INC_MULT * fabs(h) * UROUND * n * fnorm) : 1.0;
-    for (int j = 0; j < NEQ; ++j) {
-      const double yj = bcast(yv, j);
-      const double ewj = bcast(ewt, j);
-      const double inc = fmax(srur * fabs(yj), minInc / ewj);
-      const double yp = (lane == j) ? yj + inc : yv;
-      const double fp = f(yp);
-      c_nfe_dq++;
-      const double ii = 1.0 / inc;
-      fresh(jw)[j * kWave + lane] = ii * fp - ii * fy;
-    }
-  }
-
-  __device__ __forceinline__ int lsetup(int convfail) {
-    const double dgamma = fabs(gamma / gammap - 1.0);
-    const bool jbad = (nst == 0) || (nst > nstlj + MSBJ) || ((convfail == CF_BAD_J) && (dgamma < DGMAX_J)) ||
-                      (convfail == CF_OTHER);
-    if (!jbad) {
-      jcur = 0;
-    } else {
-      c_nje++;
-      nstlj = nst;
-      jcur = 1;
-      dq_jac(zn[0][lane], ftemp);
-    }
-    const double* jr = fresh(jw) + lane;
-#pragma unroll
-    for (int c = 0; c < NEQ; ++c) {
-      double v = -gamma * jr[c * kWave];
-      if (c == lane) v += 1.0;
-      M[c] = v;
-    }
-    c_nlu++;
-    return getrf() ? 1 : 0;
-  }
-
-  // ---- BDF bookkeeping ----------------------------------------------------
-  __device__ __forceinline__ void set_bdf() {
-    double xi_inv = 1.0, xistar_inv = 1.0, alpha0 = -1.0, alpha0_hat = -1.0, hsum = h;
-    l[0] = l[1] = 1.0;
-#pragma unroll
-    for (int i = 2; i <= kQmaxAbs; ++i) l[i] = 0.0;
-    if (q > 1) {
-#pragma unroll
-      for (int j = 2; j < kQmaxAbs; ++j) {
-        if (j < q) {
-          hsum += tau[j - 1];
-          xi_inv = h / hsum;
-          alpha0 -= 1.0 / j;
-#pragma unroll
-          for (int i = j; i >= 1; --i) l[i] += l[i - 1] * xi_inv;
-        }
-      }
-      alpha0 -= 1.0 / q;
-      xistar_inv = -l[1] - alpha0;
-      hsum += tau[q - 1];
-      xi_inv = h / hsum;
-      alpha0_hat = -l[1] - xi_inv;
-#pragma unroll
-      for (int i = kQmaxAbs; i >= 1; --i)
-        if (i <= q) l[i] += l[i - 1] * xistar_inv;
-    }
-    const double A1 = 1.0 - alpha0_hat + alpha0;
-    const double A2 = 1.0 + q * A1;
-    const double lq = l[q];
-    tq[2] = fabs(A1 / (alpha0 * A2));
-    tq[5] = fabs(A2 * xistar_inv / (lq * xi_inv));
-    if (qwait == 1) {
-      if (q > 1) {
-        const double C = xistar_inv / lq;
-        const double A3 = alpha0 + 1.0 / q;
-        const double A4 = alpha0_hat + xi_inv;
-        const double Cpinv = (1.0 - A4 + A3) / A3;
-        tq[1] = fabs(C * Cpinv);
-      } else {
-        tq[1] = 1.0;
-      }
-      hsum += tau[q];
-      xi_inv = h / hsum;
-      const double A5 = alpha0 - (1.0 / (q + 1));
-      const double A6 = alpha0_hat - xi_inv;
-      const double Cppinv = (1.0 - A6 + A5) / A2;
-      tq[3] = fabs(Cppinv / (xi_inv * (q + 2) * A5));
-    }
-    tq[4] = NLSCOEF / tq[2];
-  }
-
-  __device__ __forceinline__ void set_coeffs() {
-    set_bdf();
-    rl1 = 1.0 / l[1];
-    gamma = h * rl1;
-    if (nst == 0) gammap = gamma;
-    gamrat = (nst > 0) ? gamma / gammap : 1.0;
-  }
-
-  __device__ __forceinline__ void predict(double tstop) {
-    tn += h;
-    if ((tn - tstop) * h > 0.0) tn = tstop;
-#pragma unroll
-    for (int k = 1; k <= kQmaxAbs; ++k)
-#pragma unroll
-      for (int j = kQmaxAbs; j >= k; --j)
-        if (k <= q && j <= q) zn[j - 1][lane] += zn[j][lane];
-  }
-
-  __device__ __forceinline__ void restore(double saved_t) {
-    tn = saved_t;
-#pragma unroll
-    for (int k = 1; k <= kQmaxAbs; ++k)
-#pragma unroll
-      for (int j = kQmaxAbs; j >= k; --j)
-        if (k <= q && j <= q) zn[j - 1][lane] -= zn[j][lane];
-  }
-
-  __device__ __forceinline__ void rescale() {
-    double factor = eta;
-#pragma unroll
-    for (int j = 1; j <= kQmaxAbs; ++j)
-      if (j <= q) {
-        zn[j][lane] *= factor;
-        factor *= eta;
-      }
-    h = hscale * eta;
-    hscale = h;
-  }
-
-  __device__ __forceinline__ void increase_order() {
-    double lo[kQmaxAbs + 2];
-#pragma unroll
-    for (int i = 0; i < kQmaxAbs + 2; ++i) lo[i] = 0.0;
-    double alpha1 = 1.0, prod = 1.0, xiold = 1.0, alpha0 = -1.0, hsum = hscale;
-    lo[2] = 1.0;
-#pragma unroll
-    for (int j = 1; j < kQmaxAbs; ++j) {
-      if (j < q) {
-        hsum += tau[j + 1];
-        const double xi = hsum / hscale;
-        prod *= xi;
-        alpha0 -= 1.0 / (j + 1);
-        alpha1 += 1.0 / xi;
-#pragma unroll
-        for (int i = j + 2; i >= 2; --i) lo[i] = lo[i] * xiold + lo[i - 1];
-        xiold = xi;
-      }
-    }
-    const double A1 = (-alpha0 - alpha1) / prod;
-    const double zl = A1 * zn[qmax][lane];
-    zn[Lq][lane] = zl;
-#pragma unroll
-    for (int j = 2; j <= kQmaxAbs; ++j)
-      if (j <= q) zn[j][lane] += lo[j] * zl;
-  }
-
-  __device__ __forceinline__ void decrease_order() {
-    double lo[kQmaxAbs + 2];
-#pragma unroll
-    for (int i = 0; i < kQmaxAbs + 2; ++i) lo[i] = 0.0;
-    lo[2] = 1.0;
-    double hsum = 0.0;
-#pragma unroll
-    for (int j = 1; j <= kQmaxAbs - 2; ++j) {
-      if (j <= q - 2) {
-        hsum += tau[j];
-        const double xi = hsum / hscale;
-#pragma unroll
-        for (int i = j + 2; i >= 2; --i) lo[i] = lo[i] * xi + lo[i - 1];
-      }
-    }
-    const double zq = zn[q][lane];
-#pragma unroll
-    for (int j = 2; j < kQmaxAbs; ++j)
-      if (j < q) zn[j][lane] -= lo[j] * zq;
-  }
-
-  __device__ __forceinline__ void adjust_order(int dq) {
-    if (q == 2 && dq != 1) return;
-    if (dq == 1) increase_order();
-    else decrease_order();
-  }
-
-  __device__ __forceinline__ void adjust_params() {
-    if (qprime != q) {
-      adjust_order(qprime - q);
-      q = qprime;
-      Lq = q + 1;
-      qwait = Lq;
-    }
-    rescale();
-  }
-
-  __device__ __forceinline__ int newton_iter() {
-    int mm = 0;
-    double delp = 0.0;
-    for (;;) {
-      const double r = rl1 * zn[1][lane] + acor;
-      double b = act ? gamma * ftemp - r : 0.0;
-      b = getrs(b);
-      if (gamrat != 1.0) b *= 2.0 / (1.0 + gamrat);
-      c_nni++;
-      const double del = wrms(b);
-      acor += b;
-      y = zn[0][lane] + acor;
-      if (mm > 0) crate = fmax(CRDOWN * crate, del / delp);
-      const double dcon = del * fmin(1.0, crate) / tq[4];
-      if (dcon <= 1.0) {
-        acnrm = (mm == 0) ? del : wrms(acor);
-        jcur = 0;
-        return NF_OK;
-      }
-      ++mm;
-      if (mm == NLS_MAXCOR || (mm >= 2 && del > RDIV * delp)) return jcur ? NF_CONV_FAIL : NF_TRY_AGAIN;
-      delp = del;
-      ftemp = f(y);
-      c_nfe++;
-    }
-  }
-
-  __device__ __forceinline__ int nls(int nflag) {
-    int convfail = (nflag == NF_FIRST || nflag == NF_PREV_ERR) ? CF_NONE : CF_OTHER;
-    bool callSetup = (nflag == NF_PREV_CONV) || (nflag == NF_PREV_ERR) || (nst == 0) || (nst >= nstlp + MSBP) ||
-                     (fabs(gamrat - 1.0) > DGMAX);
-    for (;;) {
-      ftemp = f(zn[0][lane]);
-      c_nfe++;
-      if (callSetup) {
-        const int ier = lsetup(convfail);
-        c_nsetups++;
-        callSetup = false;
-        gamrat = crate = 1.0;
-        gammap = gamma;
-        nstlp = nst;
-        if (ier) return NF_CONV_FAIL;
-      }
-      acor = 0.0;
-      y = zn[0][lane];
-      const int ier = newton_iter();
-      if (ier != NF_TRY_AGAIN) return ier;
-      callSetup = true;
-      convfail = CF_BAD_J;
-    }
-  }
-
-  __device__ __forceinline__ int handle_nflag(int& nflag, double saved_t, int& ncf) {
-    if (nflag == NF_OK) return 0;
-    c_ncfn++;
-    restore(saved_t);
-    ncf++;
-    etamax = 1.0;
-    if (ncf == MXNCF) return TGR_ST_CONV_FAIL;
-    eta = ETACF;
-    nflag = NF_PREV_CONV;
-    rescale();
-    return 1;
-  }
-
-  __device__ __forceinline__ int error_test(int& nflag, double saved_t, int& nef, double& dsm) {
-    dsm = acnrm * tq[2];
-    if (dsm <= 1.0) return 0;
-    nef++;
-    c_netf++;
-    nflag = NF_PREV_ERR;
-    restore(saved_t);
-    if (nef == MXNEF) return TGR_ST_ERR_FAIL;
-    etamax = 1.0;
-    if (nef <= MXNEF1) {
-      eta = 1.0 / (rpow(BIAS2 * dsm, 1.0 / Lq) + ADDON);
-      eta = fmax(ETAMIN, eta);
-      if (nef >= SMALL_NEF) eta = fmin(eta, ETAMXF);
-      rescale();
-      return 1;
-    }
-    if (q > 1) {
-      eta = ETAMIN;
-      adjust_order(-1);
-      Lq = q;
-      q--;
-      qwait = Lq;
-      rescale();
-      return 1;
-    }
-    eta = ETAMIN;
-    h *= eta;
-    hscale = h;
-    qwait = LONG_WAIT;
-    tempv = f(zn[0][lane]);
-    c_nfe++;
-    zn[1][lane] = act ? h * tempv : 0.0;
-    return 1;
-  }
-
-  __device__ __forceinline__ void complete_step() {
-    nst++;
-#pragma unroll
-    for (int i = kQmaxAbs; i >= 2; --i)
-      if (i <= q) tau[i] = tau[i - 1];
-    if (q == 1 && nst > 1) tau[2] = tau[1];
-    tau[1] = h;
-#pragma unroll
-    for (int j = 0; j <= kQmaxAbs; ++j)
-      if (j <= q) zn[j][lane] += l[j] * acor;
-    qwait--;
-    if (qwait == 1 && q != qmax) {
-      zn[qmax][lane] = acor;
-      saved_tq5 = tq[5];
-    }
-  }
-
-  __device__ __forceinline__ void set_eta() {
-    if (eta < THRESH) {
-      eta = 1.0;
-      hprime = h;
-    } else {
-      eta = fmin(eta, etamax);
-      hprime = h * eta;
-    }
-  }
-
-  __device__ __forceinline__ void prepare_next(double dsm) {
-    if (etamax == 1.0) {
-      if (qwait < 2) qwait = 2;
-      qprime = q;
-      hprime = h;
-      eta = 1.0;
-      return;
-    }
-    const double etaq = 1.0 / (rpow(BIAS2 * dsm, 1.0 / Lq) + ADDON);
-    if (qwait != 0) {
-      eta = etaq;
-      qprime = q;
-      set_eta();
-      return;
-    }
-    qwait = 2;
-    double etaqm1 = 0.0;
-    if (q > 1) {
-      const double ddn = wrms(zn[q][lane]) * tq[1];
-      etaqm1 = 1.0 / (rpow(BIAS1 * ddn, 1.0 / q) + ADDON);
-    }
-    double etaqp1 = 0.0;
-    if (q != qmax && saved_tq5 != 0.0) {
-      const double cquot = (tq[5] / saved_tq5) * rpow(h / tau[2], (double)Lq);
-      const double tv = acor - cquot * zn[qmax][lane];
-      const double dup = wrms(tv) * tq[3];
-      etaqp1 = 1.0 / (rpow(BIAS3 * dup, 1.0 / (Lq + 1)) + ADDON);
-    }
-    const double etam = fmax(etaqm1, fmax(etaq, etaqp1));
-    if (etam < THRESH) {
-      eta = 1.0;
-      qprime = q;
-    } else if (etam == etaq) {
-      eta = etaq;
-      qprime = q;
-    } else if (etam == etaqm1) {
-      eta = etaqm1;
-      qprime = q - 1;
-    } else {
-      eta = etaqp1;
-      qprime = q + 1;
-      zn[qmax][lane] = acor;
-    }
-    set_eta();
-  }
-
-  __device__ __forceinline__ int step(double tstop) {
-    const double saved_t = tn;
-    double dsm = 0.0;
-    int ncf = 0, nef = 0, nflag = NF_FIRST;
-    if (nst > 0 && hprime != h) adjust_params();
-    for (;;) {
-      predict(tstop);
-      set_coeffs();
-      nflag = nls(nflag);
-      const int k = handle_nflag(nflag, saved_t, ncf);
-      if (k == 1) continue;
-      if (k < 0) return k;
-      const int e = error_test(nflag, saved_t, nef, dsm);
-      if (e == 1) continue;
-      if (e < 0) return e;
-      break;
-    }
-    complete_step();
-    prepare_next(dsm);
-    etamax = (nst <= SMALL_NST) ? ETAMX2 : ETAMX3;
-    acor *= 1.0 / tq[2];
-    return 0;
-  }
-
-  __device__ __forceinline__ double initial_h(double tout) {
-    const double tdist = fabs(tout - tn);
-    const double tround = UROUND * fmax(fabs(tn), fabs(tout));
-    const double hlb = HLB_FACTOR * tround;
-    const double den = HUB_FACTOR * fabs(zn[0][lane]) + 1.0 / ewt;
-    const double hub_inv = (wave_max(act ? fabs(zn[1][lane]) / den : 0.0));
-    double hub = HUB_FACTOR * tdist;
-    if (hub * hub_inv > 1.0) hub = 1.0 / hub_inv;
-    double hg = sqrt(hlb * hub);
-    if (hub < hlb) return hg;
-    double hnew = hg;
-    bool hnewOK = false;
-    for (int it = 1; it <= HIN_ITERS; ++it) {
-      const double yt = hg * zn[1][lane] + zn[0][lane];
-      double fv = f(yt);
-      c_nfe++;
-      fv = (fv - zn[1][lane]) * (1.0 / hg);
-      const double yddnrm = wrms(fv);
-      if (hnewOK || it == HIN_ITERS) {
-        hnew = hg;
-        break;
-      }
-      hnew = (yddnrm * hub * hub > 2.0) ? sqrt(2.0 / yddnrm) : sqrt(hg * hub);
-      const double hrat = hnew / hg;
-      if (hrat > 0.5 && hrat < 2.0) hnewOK = true;
-      if (it > 1 && hrat > 2.0) {
-        hnew = hg;
-        hnewOK = true;
-      }
-      hg = hnew;
-    }
-    double h0 = H_BIAS * hnew;
-    if (h0 < hlb) h0 = hlb;
-    if (h0 > hub) h0 = hub;
-    return h0;
-  }
-
-  // integrate 0 -> tout; sol: this lane's component, updated in place
-  __device__ __forceinline__ int integrate(double tout, double& sol) {
-    c_nfe = c_nfe_dq = c_nje = c_nsetups = c_nni = c_netf = c_ncfn = c_nlu = 0;
-    nst = 0;
-    const bool finite = isfinite(act ? sol : 0.0);
-    if (__ballot(!finite) != 0ull) return TGR_ST_BAD_INPUT;
-    tn = 0.0;
-    q = 1;
-    Lq = 2;
-    qwait = Lq;
-    qprime = 1;
-    etamax = ETAMX1;
-    nstlp = nstlj = 0;
-    saved_tq5 = 0.0;
-    crate = 1.0;
-    gammap = gamrat = 1.0;
-    jcur = 0;
-    eta = 1.0;
-#pragma unroll
-    for (int j = 0; j < kQmaxAbs + 2; ++j) tau[j] = 0.0;
-#pragma unroll
-    for (int j = 0; j < 6; ++j) tq[j] = 0.0;
-#pragma unroll
-    for (int j = 0; j <= kQmaxAbs; ++j) zn[j][lane] = 0.0;
-#pragma unroll
-    for (int c = 0; c < NEQ; ++c) M[c] = 0.0;
-    pstep = -1;
-    rdiag = 0.0;
-    acor = y = tempv = 0.0;
-    zn[0][lane] = act ? sol : 0.0;
-    set_ewt(zn[0][lane]);
-    zn[1][lane] = f(zn[0][lane]);
-    c_nfe++;
-    const double tstop = tout;
-    if (!(fabs(tout - tn) >= 2.0 * UROUND * fabs(tout)) || tout <= 0.0) return TGR_ST_BAD_INPUT;
-    h = initial_h(tout);
-    if ((tn + h - tstop) * h > 0.0) h = (tstop - tn) * (1.0 - 4.0 * UROUND);
-    hscale = h;
-    hprime = h;
-    zn[1][lane] *= h;
-    int nstloc = 0;
-    for (;;) {
-      if (nst > 0) set_ewt(zn[0][lane]);
-      if (nstloc >= max_steps) {
-        sol = zn[0][lane];
-        return TGR_ST_TOO_MUCH_WORK;
-      }
-      if (UROUND * wrms(zn[0][lane]) > 1.0) {
-        sol = zn[0][lane];
-        return TGR_ST_TOO_MUCH_ACC;
-      }
-      const int k = step(tstop);
-      if (k < 0) {
-        sol = zn[0][lane];
-        return k;
-      }
-      nstloc++;
-      double tret = -1.0;
-      if ((tn - tout) * h >= 0.0) {
-        tret = tout;
-      } else {
-        const double troundoff = FUZZ_FACTOR * UROUND * (fabs(tn) + fabs(h));
-        if (fabs(tn - tstop) <= troundoff) tret = tstop;
-      }
-      if (tret >= 0.0) {
-        const double s = (tret - tn) / h;
-        double v = zn[q][lane];
-#pragma unroll
-        for (int j = kQmaxAbs - 1; j >= 0; --j)
-          if (j <= q - 1) v = zn[j][lane] + s * v;
-        sol = v;
-        return TGR_ST_OK;
-      }
-      if ((tn + hprime - tstop) * h > 0.0) {
-        hprime = (tstop - tn) * (1.0 - 4.0 * UROUND);
-        eta = hprime / h;
-      }
-    }
-  }
-};
-
-// ---------------------------------------------------------------------------
-// finitechem_source_chemistry on a Fortran-layout grid
-// ---------------------------------------------------------------------------
+// finitechem_source_chemistry on a Fortran-layout block.  One wavefront per
+// workgroup; workgroups pull cells from a device counter until all are taken,
+// so that cells whose step counts differ by 10-100x keep every wave busy.
 template <int NEQ>
 __global__ __launch_bounds__(64) void step_kernel(StepParams p) {
   __shared__ RhsLds<NEQ> lds;
   __shared__ double zn_lds[kQmaxAbs + 2][kWave];  // Nordsieck columns + tq, l, tau
   const DevMech m = p.m;
   const int lane = lane_id();
-  const int nsp = m.nsp;
+  double* ws = p.ws + (int64_t)blockIdx.x * NEQ * kWave;
   for (;;) {
     unsigned int c = 0;
     if (lane == 0) c = atomicAdd(p.queue, 1u);
     const int64_t cell = (int64_t)(unsigned int)__builtin_amdgcn_readfirstlane((int)c);
     if (cell >= p.ncell) break;
-    const int i = (int)(cell % p.nx);
-    const int j = (int)((cell / p.nx) % p.ny);
-    const int k = (int)(cell / ((int64_t)p.nx * p.ny));
-    bool skip = false;
-    if (p.mask) skip = p.mask[p.mask_base + (int64_t)j * p.mask_ld + i] == 1;
-    if (!skip) {
-      const int64_t off = p.base + i * p.sx + j * p.sy + k * p.sz;
-      const double rho = p.rho[off];
-      double sol = 0.0;
-      if (lane < nsp) sol = fmax(rho * p.sc[(int64_t)(p.isc_sc0 + lane) * p.plane + off], 0.0);
-      else if (lane == nsp) sol = rho * p.sc[(int64_t)p.isc_T0 * p.plane + off];
-      const double sol0 = sol;
-      Bdf<NEQ> b(m, lds, zn_lds, p.ws + (int64_t)blockIdx.x * NEQ * kWave, p.p_gp, p.rtol, p.atol, p.qmax,
-                 p.max_steps);
-      const int st = b.integrate(p.dt, sol);
-      // unpack (finitechem.f90:1041-1042) and diagnostics (1047-1058)
-      const double v = sol / rho;
-      if (lane < nsp) p.sc[(int64_t)(p.isc_sc0 + lane) * p.plane + off] = v;
-      else if (lane == nsp) p.sc[(int64_t)p.isc_T0 * p.plane + off] = v;
-      const double Tn = bcast(v, nsp);
-      double h, cp;
-      thermo_lane<NEQ>(fresh(m.tab), m.tmid, Tn, lane, h, cp);
-      const double cpm = (wave_sum(lane < nsp ? v * cp : 0.0));
-      if (p.hr && lane == nsp) p.hr[off] = cpm * (sol - sol0) / p.dt;
-      if (p.chemsrc) {
-        const double src = (sol - sol0) / p.dt;
-        const int sp3[3] = {p.idx_h2o, p.idx_co, p.idx_co2};
+    int cnt[TGR_NCOUNT];
+    if (!process_cell<NEQ>(p, m, cell, lds, zn_lds, ws, cnt)) continue;
+    if (lane == 0) {
+      if (p.cell_stats) {
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          if (sp3[q] >= 0 && lane == sp3[q]) p.chemsrc[q * p.plane + off] = src;
-          if (sp3[q] < 0 && lane == nsp) p.chemsrc[q * p.plane + off] = 0.0;
-        }
+        for (int q = 0; q < TGR_NCOUNT; ++q) p.cell_stats[cell * TGR_NCOUNT + q] = cnt[q];
       }
-      if (lane == 0) {
-        const int cnt[TGR_NCOUNT] = {b.nst, b.c_nfe, b.c_nfe_dq, b.c_nje, b.c_nsetups,
-                                     b.c_nni, b.c_netf, b.c_ncfn, b.c_nlu, st};
-        if (p.cell_stats) {
+      if (p.totals) {
 #pragma unroll
-          for (int c = 0; c < TGR_NCOUNT; ++c) p.cell_stats[cell * TGR_NCOUNT + c] = cnt[c];
-        }
-        if (p.totals) {
-#pragma unroll
-          for (int c = 0; c < TGR_NCOUNT - 1; ++c) atomicAdd(&p.totals[c], (unsigned long long)cnt[c]);
-          if (st != 0) atomicAdd(&p.totals[TGR_NCOUNT - 1], 1ull);
-        }
+        for (int q = 0; q < TGR_NCOUNT - 1; ++q) atomicAdd(&p.totals[q], (unsigned long long)cnt[q]);
+        if (cnt[TGR_NCOUNT - 1] != 0) atomicAdd(&p.totals[TGR_NCOUNT - 1], 1ull);
       }
     }
   }

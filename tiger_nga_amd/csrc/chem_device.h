@@ -24,7 +24,7 @@
 // single base register plus lane offset plus immediate: nothing per-table has
 // to stay live in registers across the solver loop.
 #pragma once
-#include <hip/hip_runtime.h>
+#include "wave_target.h"
 #include <stdint.h>
 
 #include "fp64_math.h"
@@ -166,7 +166,7 @@ __device__ __forceinline__ double wave_max(double v) {
 // values live in registers across the whole integration.
 template <class T>
 __device__ __forceinline__ T* fresh(T* p) {
-  asm volatile("" : "+s"(p));
+  asm volatile("" : TGR_OPAQUE_PTR(p));
   return p;
 }
 

@@ -14,7 +14,7 @@
 //   e ln2_hi + (2s P(z) + e ln2_lo).  x = 0 gives -inf, x < 0 NaN, +inf +inf.
 //   Ops: 2 frexp, 1 cmp/select pair, 2 add, 1 div, 1 mul, 13 fma.
 #pragma once
-#include <hip/hip_runtime.h>
+#include "wave_target.h"
 
 namespace tgr {
 
