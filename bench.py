@@ -32,7 +32,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="gri30", choices=["h2", "gri30", "gri30_uneven", "dme39"])
-    ap.add_argument("--cpu-cells", type=int, default=8192)
+    ap.add_argument("--cpu-cells", type=int, default=131072)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all usable host cores, at most 16")
     ap.add_argument("--no-cpu", action="store_true")
     return ap.parse_args()

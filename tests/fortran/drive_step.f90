@@ -1,0 +1,53 @@
+! Fortran caller of the chemistry step through the ISO_C_BINDING module, as a
+! Tiger_NGA rank would call it in place of finitechem_source_chemistry.
+! Usage: drive_step <mech> <in.bin> <out.bin>
+!   in.bin  (stream): int32 nx ny nz nover nscalar isc_sc isc_T; float64 dt P rtol atol;
+!           RHO_(nxo,nyo,nzo); SC_(nxo,nyo,nzo,nscalar); int32 mask(nxo,nyo)
+!   out.bin (stream): int32 ierr; SC_; HR_gp; chemsrc_gp(...,3); int64 stats(11)
+program drive_step
+  use iso_c_binding
+  use tiger_chem
+  implicit none
+  character(len=64) :: mech
+  character(len=512) :: fin, fout
+  integer(c_int) :: hdr(7)
+  integer :: nx, ny, nz, nover, nscalar, ierr, u
+  real(c_double) :: dt, P, rtol, atol
+  real(c_double), allocatable, target :: RHO_(:,:,:), SC_(:,:,:,:), HR_gp(:,:,:), chemsrc_gp(:,:,:,:)
+  integer(c_int), allocatable, target :: mask(:,:)
+  type(tgr_grid) :: g
+  type(tgr_stats) :: st
+
+  call get_command_argument(1, mech)
+  call get_command_argument(2, fin)
+  call get_command_argument(3, fout)
+  open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+  read(u) hdr
+  nx = hdr(1); ny = hdr(2); nz = hdr(3); nover = hdr(4); nscalar = hdr(5)
+  read(u) dt, P, rtol, atol
+  allocate(RHO_(1-nover:nx+nover, 1-nover:ny+nover, 1-nover:nz+nover))
+  allocate(SC_(1-nover:nx+nover, 1-nover:ny+nover, 1-nover:nz+nover, nscalar))
+  allocate(HR_gp(1-nover:nx+nover, 1-nover:ny+nover, 1-nover:nz+nover))
+  allocate(chemsrc_gp(1-nover:nx+nover, 1-nover:ny+nover, 1-nover:nz+nover, 3))
+  allocate(mask(1-nover:nx+nover, 1-nover:ny+nover))
+  read(u) RHO_, SC_, mask
+  close(u)
+  HR_gp = 0.0_c_double
+  chemsrc_gp = 0.0_c_double
+
+  g = tgr_grid(1, nx, 1, ny, 1, nz, 1-nover, nx+nover, 1-nover, ny+nover, 1-nover, nz+nover, &
+       1-nover, nx+nover, 1-nover, ny+nover, nscalar, hdr(6), hdr(7))
+  call tiger_chem_init(mech, rtol, atol, ierr)
+  if (ierr == 0) call tiger_chem_source_chemistry(RHO_, SC_, g, dt, P, P, ierr, mask, HR_gp, chemsrc_gp, st)
+  if (ierr /= 0) then
+     write(*, '(a, i0, 2a)') 'tiger_chem error ', ierr, ': ', trim(tiger_chem_message())
+     st = tgr_stats(0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0.0_c_double)
+  end if
+  call tiger_chem_final()
+
+  open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+  write(u) int(ierr, c_int), SC_, HR_gp, chemsrc_gp
+  write(u) st%cells, st%nst, st%nfe, st%nfe_dq, st%nje, st%nsetups, st%nni, st%netf, st%ncfn, st%nlu, st%failed
+  close(u)
+  if (ierr /= 0) stop 1
+end program drive_step

@@ -75,3 +75,37 @@ def test_mass_and_element_conservation_of_rates(mech, oracle):
         assert abs((W * cd_o).sum()) <= 1e-6 * max(scale, 1e-300)   # binary32 molar masses
         assert np.abs(E @ cd_o).max() <= 1e-12 * max(np.abs(Wsl).sum() * 6, 1e-300)
 
+
+
+def _ref_lib(mech):
+    path = os.path.join(os.path.dirname(GOLD), "..", "oracle", "_ref", "libref_%s.so" % mech)
+    if not os.path.exists(path):
+        pytest.skip("oracle/_ref not built (reference mechanism sources absent)")
+    import ctypes
+    return ctypes.CDLL(os.path.abspath(path), mode=os.RTLD_LOCAL)
+
+
+@pytest.mark.parametrize("mech", MECHS)
+def test_oracle_vs_reference_mechanism_random_states(mech, oracle):
+    """Beyond the committed vectors: 500 fresh states per mechanism through the
+    reference mechanism file itself (oracle/_ref, compiled from the reference
+    sources by oracle/Makefile) and the oracle, K and W bit for bit."""
+    import ctypes
+    lib = _ref_lib(mech)
+    dp = ctypes.POINTER(ctypes.c_double)
+    d = _gold(mech)
+    nsp, nslot = len(d["names"]), int(d["nslot"])
+    rng = np.random.default_rng(99)
+    for s in range(500):
+        T = rng.uniform(280.0, 3200.0)
+        P = rng.choice([101325.0, 1013250.0, 5e4])
+        C = 10.0 ** rng.uniform(-14.0, -1.0, nsp)
+        C[rng.random(nsp) < 0.1] = 0.0
+        cd, w, k, m = np.zeros(nsp), np.zeros(nslot), np.zeros(nslot), np.zeros(64)
+        c = C.copy()
+        lib.prodrates_(cd.ctypes.data_as(dp), w.ctypes.data_as(dp), k.ctypes.data_as(dp), c.ctypes.data_as(dp),
+                       m.ctypes.data_as(dp), ctypes.byref(ctypes.c_double(T)), ctypes.byref(ctypes.c_double(P)))
+        cd_o, K_o, W_o, _ = oracle.prodrates(mech, C, T, P, nslot)
+        np.testing.assert_array_equal(K_o[:nslot], k)
+        np.testing.assert_array_equal(W_o[:nslot], w)
+        assert np.abs(cd_o - cd).max() <= 1e-14 * max(np.abs(w).sum(), 1e-300)

@@ -1,4 +1,5 @@
-"""Builds libtiger_chem.so in-tree for gfx950 with hipcc.
+"""Builds libtiger_chem.so in-tree for gfx950 with hipcc, then the Fortran
+ISO_C_BINDING module and the Fortran test caller with flang.
 
 Flags: -O3, C++17, --offload-arch=gfx950, and packed-FP32 instruction
 selection disabled for device code (-Xclang -target-feature -Xclang
@@ -23,6 +24,33 @@ def command(out=OUT, extra=()):
             *[os.path.join(CSRC, s) for s in SOURCES], "-o", out]
 
 
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FORTRAN_SRC = os.path.join(HERE, "fortran", "tiger_chem_mod.f90")
+FORTRAN_OUT = os.path.join(HERE, "fortran", "_build")
+DRIVER_SRC = os.path.join(HERE, "..", "tests", "fortran", "drive_step.f90")
+DRIVER_OUT = os.path.join(HERE, "..", "tests", "fortran", "_build", "drive_step")
+
+
+def build_fortran(verbose=False):
+    """Compiles the ISO_C_BINDING module (tiger_chem.mod + object) and the
+    Fortran test caller linked against libtiger_chem.so."""
+    if not os.path.exists(FLANG):
+        return None
+    os.makedirs(FORTRAN_OUT, exist_ok=True)
+    obj = os.path.join(FORTRAN_OUT, "tiger_chem_mod.o")
+    cmds = [[FLANG, "-O2", "-fPIC", "-c", FORTRAN_SRC, "-module-dir", FORTRAN_OUT, "-o", obj]]
+    if os.path.exists(DRIVER_SRC):
+        os.makedirs(os.path.dirname(DRIVER_OUT), exist_ok=True)
+        rel = os.path.relpath(HERE, os.path.dirname(DRIVER_OUT))
+        cmds.append([FLANG, "-O2", DRIVER_SRC, obj, "-I", FORTRAN_OUT, "-L" + HERE, "-ltiger_chem",
+                     "-Wl,-rpath,$ORIGIN/" + rel, "-o", DRIVER_OUT])
+    for c in cmds:
+        if verbose:
+            print(" ".join(c))
+        subprocess.check_call(c)
+    return DRIVER_OUT
+
+
 def build(verbose=False):
     tmp = OUT + ".tmp"
     cmd = command(tmp)
@@ -30,6 +58,7 @@ def build(verbose=False):
         print(" ".join(cmd))
     subprocess.check_call(cmd)
     os.replace(tmp, OUT)
+    build_fortran(verbose)
     return OUT
 
 
