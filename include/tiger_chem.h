@@ -108,6 +108,8 @@ int tgr_chem_rhs_device(tgr_chem *ctx, int64_t n, double p_gp, const double *sol
 /* Tuning knobs (no counterpart in the reference):
  *   "grid"        size of the persistent grid in blocks (0: occupancy x CUs); blocks pull cells
  *                 from a device work counter
+ *   "rhs_mapping" tgr_chem_rhs_device: 1 (default) one state per lane (generated
+ *                 straight-line code of the mechanism), 0 one state per wavefront
  *   "max_order"   BDF maximum order 1..5 (CVODE default 5)
  *   "max_steps"   MAX_NSTEPS */
 int tgr_chem_set_option(tgr_chem *ctx, const char *key, int64_t value);

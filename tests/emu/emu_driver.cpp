@@ -7,10 +7,14 @@
 // check the GPU solver logic against the C oracle without a GPU, and with
 // sanitizers for bounds checks of the device code.
 #include <cstring>
+#include <type_traits>
 #include <string>
 #include <vector>
 
 #include "bdf_wave.h"
+#include "lane_rhs_dme39.h"
+#include "lane_rhs_gri30.h"
+#include "lane_rhs_h2.h"
 #include "mech_dme39.h"
 #include "mech_gri30.h"
 #include "mech_h2.h"
@@ -35,7 +39,6 @@ template <int NEQ>
 struct Run {
   const tgr::StepParams* p;
   int* counters;
-  tgr::RhsLds<NEQ> lds;
   // per-lane copy of the LDS history block: every lane reads and writes only
   // its own column, plus one wave-uniform row that all lanes update with the
   // same values -- one update in SIMT, but sequential fibers sharing it would
@@ -49,7 +52,7 @@ struct Run {
     Run* r = (Run*)(((uintptr_t)hi << 32) | (uintptr_t)lo);
     for (int64_t c = 0; c < r->p->ncell; ++c) {
       int cnt[TGR_NCOUNT] = {0};
-      const int done = tgr::process_cell<NEQ>(*r->p, r->p->m, c, r->lds, r->zn[l], r->ws.data(), cnt);
+      const int done = tgr::process_cell<NEQ>(*r->p, r->p->m, c, r->zn[l], r->ws.data(), cnt);
       if (l == 0 && r->counters)
         for (int q = 0; q < TGR_NCOUNT; ++q) r->counters[c * TGR_NCOUNT + q] = done ? cnt[q] : 0;
     }
@@ -130,6 +133,29 @@ extern "C" int emu_step(const char* mech, long ncell, double* rho, double* sc, d
     case 10: run<10>(p, counters); break;
     case 37: run<37>(p, counters); break;
     case 40: run<40>(p, counters); break;
+    default: return -3;
+  }
+  return 0;
+}
+
+// The generated lane-per-state right-hand side on n packed states
+// sol[n][neq] -> rhs[n][neq] (plain C++, one state at a time).
+extern "C" int emu_lane_rhs(const char* mech, long n, double p_gp, const double* sol, double* rhs) {
+  const tgr_mech_tables* t = find_mech(mech);
+  if (!t) return -1;
+  auto run_n = [&](auto tag) {
+    constexpr int NEQ = decltype(tag)::value;
+    for (long c = 0; c < n; ++c) {
+      double s[NEQ], f[NEQ];
+      for (int k = 0; k < NEQ; ++k) s[k] = sol[c * NEQ + k];
+      tgr::LaneRhs<NEQ>::eval(s, p_gp, f);
+      for (int k = 0; k < NEQ; ++k) rhs[c * NEQ + k] = f[k];
+    }
+  };
+  switch (t->nsp + 1) {
+    case 10: run_n(std::integral_constant<int, 10>()); break;
+    case 37: run_n(std::integral_constant<int, 37>()); break;
+    case 40: run_n(std::integral_constant<int, 40>()); break;
     default: return -3;
   }
   return 0;

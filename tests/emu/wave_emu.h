@@ -23,6 +23,8 @@
 #define __global__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
+// function-scope LDS: one object shared by the lanes (fibers) of the wave
+#define __shared__ static
 
 namespace tgr_emu {
 

@@ -51,12 +51,15 @@ def test_prodrates_vs_reference_golden(mech, chem, torch_dev):
     assert err.max() < 1e-12
 
 
+@pytest.mark.parametrize("mapping", [0, 1])
 @pytest.mark.parametrize("mech,workload", [("h2", "h2"), ("gri30", "gri30"), ("dme39", "dme39")])
-def test_rhs_vs_oracle(mech, workload, chem, oracle, torch_dev):
+def test_rhs_vs_oracle(mech, workload, mapping, chem, oracle, torch_dev):
+    """Batched right-hand side, wave per state (0) and lane per state (1)."""
     torch = torch_dev
-    _, Y, T, rho, P, _ = inputs.states(workload, n=256, seed=7)
+    _, Y, T, rho, P, _ = inputs.states(workload, n=300, seed=7)
     sol = inputs.packed_state(Y, T, rho)
     fc = chem(mech)
+    fc.set_option("rhs_mapping", mapping)
     out = fc.compute_rhs(torch.tensor(np.ascontiguousarray(sol.T), device="cuda"), P).cpu().numpy().T
     W = meta.molar_mass(mech)
     worst = 0.0
@@ -71,7 +74,7 @@ def test_rhs_vs_oracle(mech, workload, chem, oracle, torch_dev):
         e_sp = np.abs(out[c, :-1] - ref[:-1]).max() / scale_sp
         e_T = abs(out[c, -1] - ref[-1]) / scale_T
         worst = max(worst, e_sp, e_T)
-    print("%s rhs max scaled err %.3e" % (mech, worst))
+    print("%s rhs (mapping %d) max scaled err %.3e" % (mech, mapping, worst))
     assert worst < 1e-12
 
 

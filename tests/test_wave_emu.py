@@ -23,11 +23,12 @@ def emu():
     vp = ctypes.c_void_p
     lib.emu_step.argtypes = [ctypes.c_char_p, ctypes.c_long, vp, vp, ctypes.c_double, ctypes.c_double,
                              ctypes.c_double, ctypes.c_double, ctypes.c_int, vp, vp, vp]
+    lib.emu_lane_rhs.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_double, vp, vp]
     return lib
 
 
 @pytest.mark.parametrize("workload,n,rtol,atol", [("h2", 6, 1e-6, 1e-12), ("h2", 2, 1e-10, 1e-16),
-                                                  ("gri30", 2, 1e-6, 1e-12)])
+                                                  ("gri30", 2, 1e-6, 1e-12), ("dme39", 1, 1e-6, 1e-12)])
 def test_emulated_kernel_matches_oracle(emu, oracle, workload, n, rtol, atol):
     mech, Y, T, rho, P, dt = inputs.states(workload, n=n, seed=5)
     sc = np.ascontiguousarray(np.concatenate([Y.T, T[None, :]], axis=0))
@@ -48,3 +49,29 @@ def test_emulated_kernel_matches_oracle(emu, oracle, workload, n, rtol, atol):
     assert ew[same].max(initial=0.0) <= 0.1 and ew.max() <= 10.0, (ew, same)
     dT = np.abs(sc[-1] - ref[:, -1] / rho)
     assert dT[same].max(initial=0.0) < 1e-6 and dT.max() < 1e-2
+
+
+@pytest.mark.parametrize("workload", ["h2", "gri30", "dme39"])
+def test_generated_lane_rhs_matches_oracle(emu, oracle, workload):
+    """The generated lane-per-state right-hand side (lane_rhs_<mech>.h) against
+    the oracle's table-driven one, error relative to the gross rates."""
+    mech, Y, T, rho, P, dt = inputs.states(workload, n=64, seed=11)
+    sol = np.ascontiguousarray(inputs.packed_state(Y, T, rho))
+    out = np.zeros_like(sol)
+    assert emu.emu_lane_rhs(mech.encode(), sol.shape[0], P, sol.ctypes.data, out.ctypes.data) == 0
+    nsp = sol.shape[1] - 1
+    for c in range(sol.shape[0]):
+        ref = oracle.rhs(mech, sol[c], P)
+        C = sol[c, :nsp] / oracle_molar_mass(oracle, mech)
+        _, _, W, _ = oracle.prodrates(mech, C, sol[c, nsp] / sol[c, :nsp].sum(), P)
+        scale = np.abs(W).sum() * oracle_molar_mass(oracle, mech).max()
+        assert np.abs(out[c, :nsp] - ref[:nsp]).max() <= 1e-13 * scale
+        assert abs(out[c, nsp] - ref[nsp]) <= 1e-11 * max(abs(ref[nsp]), 1e-6 * scale)
+
+
+def oracle_molar_mass(oracle, mech):
+    import ctypes
+    n = {"h2": 9, "gri30": 36, "dme39": 39}[mech]
+    W = np.zeros(n)
+    oracle.lib().orc_molar_mass(oracle.MECH_ID[mech], W.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return W

@@ -58,7 +58,6 @@ __device__ __forceinline__ void put(T (&a)[N], int i, T v) {
 template <int NEQ>
 struct Bdf {
   const DevMech m;  // by value: fields stay in scalar registers
-  RhsLds<NEQ>& L;
   double p_gp, rtol, atol;
   int lane, n, qmax, max_steps;
   bool act;
@@ -80,9 +79,9 @@ struct Bdf {
   int q, qprime, qwait, Lq, jcur, nst, nstlp, nstlj;
   int c_nfe, c_nfe_dq, c_nje, c_nsetups, c_nni, c_netf, c_ncfn, c_nlu;
 
-  __device__ Bdf(const DevMech& mech, RhsLds<NEQ>& lds, double (&znl)[kQmaxAbs + 2][kWave], double* ws, double pgp,
+  __device__ Bdf(const DevMech& mech, double (&znl)[kQmaxAbs + 2][kWave], double* ws, double pgp,
                  double rt, double at, int qm, int ms)
-      : m(mech), L(lds), p_gp(pgp), rtol(rt), atol(at), max_steps(ms), zn(znl), jw(ws) {
+      : m(mech), p_gp(pgp), rtol(rt), atol(at), max_steps(ms), zn(znl), jw(ws) {
     tq = &znl[kQmaxAbs + 1][0];
     l = tq + 8;
     tau = tq + 16;
@@ -92,7 +91,7 @@ struct Bdf {
     qmax = (qm >= 1 && qm <= kQmaxAbs) ? qm : kQmaxAbs;
   }
 
-  __device__ __forceinline__ double f(double yv) { return rhs_wave<NEQ>(m, L, act ? yv : 0.0, p_gp); }
+  __device__ __forceinline__ double f(double yv) { return rhs_call<NEQ>(m.tab, m.tmid, act ? yv : 0.0, p_gp); }
 
   __device__ __forceinline__ double wrms(double x) {
     double v = act ? x * ewt : 0.0;
@@ -119,13 +118,12 @@ struct Bdf {
         rdiag = rp;
       }
       const bool elim = cand && lane != p;
-      const double lf = M[k] * rp;
+      // multiplier, 0 on rows that are not eliminated (then the update
+      // below leaves them unchanged exactly)
+      const double lf = elim ? M[k] * rp : 0.0;
       if (elim) M[k] = lf;
 #pragma unroll
-      for (int c = k + 1; c < NEQ; ++c) {
-        const double u = bcast(M[c], p);
-        if (elim) M[c] -= u * lf;
-      }
+      for (int c = k + 1; c < NEQ; ++c) M[c] = fma(-bcast(M[c], p), lf, M[c]);
     }
     return 0;
   }
@@ -180,7 +178,7 @@ struct Bdf {
       jcur = 1;
       dq_jac(zn[0][lane], ftemp);
     }
-    const double* jr = fresh(jw) + lane;
+    const TGR_GLOBAL double* jr = fresh(jw) + lane;
 #pragma unroll
     for (int c = 0; c < NEQ; ++c) {
       double v = -gamma * jr[c * kWave];
@@ -665,8 +663,7 @@ struct Bdf {
 // RHO_, heat release and H2O/CO/CO2 sources.  Returns 0 for a wall cell
 // (mask == 1, skipped), else 1 with the integrator counters in cnt.
 template <int NEQ>
-__device__ __forceinline__ int process_cell(const StepParams& p, const DevMech& m, int64_t cell, RhsLds<NEQ>& lds,
-                                          double (&zn_lds)[kQmaxAbs + 2][kWave], double* ws, int* cnt) {
+__device__ __forceinline__ int process_cell(const StepParams& p, const DevMech& m, int64_t cell, double (&zn_lds)[kQmaxAbs + 2][kWave], double* ws, int* cnt) {
   const int lane = lane_id();
   const int nsp = NEQ - 1;
   const int i = (int)(cell % p.nx);
@@ -682,8 +679,7 @@ __device__ __forceinline__ int process_cell(const StepParams& p, const DevMech& 
     if (lane < nsp) sol = fmax(rho * p.sc[(int64_t)(p.isc_sc0 + lane) * p.plane + off], 0.0);
     else if (lane == nsp) sol = rho * p.sc[(int64_t)p.isc_T0 * p.plane + off];
     const double sol0 = sol;
-    Bdf<NEQ> b(m, lds, zn_lds, ws, p.p_gp, p.rtol, p.atol, p.qmax,
-               p.max_steps);
+    Bdf<NEQ> b(m, zn_lds, ws, p.p_gp, p.rtol, p.atol, p.qmax, p.max_steps);
     const int st = b.integrate(p.dt, sol);
     // unpack (finitechem.f90:1041-1042) and diagnostics (1047-1058)
     const double v = sol / rho;

@@ -5,6 +5,9 @@
 
 #include "bdf_wave.h"
 #include "chem_kernels.h"
+#include "lane_rhs_dme39.h"
+#include "lane_rhs_gri30.h"
+#include "lane_rhs_h2.h"
 
 namespace tgr {
 
@@ -12,8 +15,7 @@ namespace tgr {
 // workgroup; workgroups pull cells from a device counter until all are taken,
 // so that cells whose step counts differ by 10-100x keep every wave busy.
 template <int NEQ>
-__global__ __launch_bounds__(64) void step_kernel(StepParams p) {
-  __shared__ RhsLds<NEQ> lds;
+__global__ __launch_bounds__(64, 2) void step_kernel(StepParams p) {
   __shared__ double zn_lds[kQmaxAbs + 2][kWave];  // Nordsieck columns + tq, l, tau
   const DevMech m = p.m;
   const int lane = lane_id();
@@ -24,7 +26,7 @@ __global__ __launch_bounds__(64) void step_kernel(StepParams p) {
     const int64_t cell = (int64_t)(unsigned int)__builtin_amdgcn_readfirstlane((int)c);
     if (cell >= p.ncell) break;
     int cnt[TGR_NCOUNT];
-    if (!process_cell<NEQ>(p, m, cell, lds, zn_lds, ws, cnt)) continue;
+    if (!process_cell<NEQ>(p, m, cell, zn_lds, ws, cnt)) continue;
     if (lane == 0) {
       if (p.cell_stats) {
 #pragma unroll
@@ -66,6 +68,21 @@ __global__ __launch_bounds__(64) void rhs_kernel(DevMech m, int64_t n, double p_
   if (lane < NEQ) f[lane * n + c] = v;
 }
 
+// Lane per state: each lane evaluates a whole right-hand side with the
+// generated code of the mechanism (lane_rhs_<mech>.h); component-major
+// arrays, so lane c's loads and stores of component k are coalesced.
+template <int NEQ>
+__global__ __launch_bounds__(64) void rhs_lanes_kernel(int64_t n, double p_gp, const double* y, double* f) {
+  const int64_t c = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (c >= n) return;
+  double s[NEQ], o[NEQ];
+#pragma unroll
+  for (int k = 0; k < NEQ; ++k) s[k] = y[k * n + c];
+  LaneRhs<NEQ>::eval(s, p_gp, o);
+#pragma unroll
+  for (int k = 0; k < NEQ; ++k) f[k * n + c] = o[k];
+}
+
 __global__ void reset_counter_kernel(unsigned int* q) { *q = 0u; }
 
 }  // namespace tgr
@@ -103,6 +120,18 @@ extern "C" int tgr_launch_prodrates(const DevMech* m, int64_t n, const double* C
     case 10: hipLaunchKernelGGL(prodrates_kernel<10>, dim3((unsigned)n), dim3(64), 0, s, *m, n, C, T, P, cdot); break;
     case 37: hipLaunchKernelGGL(prodrates_kernel<37>, dim3((unsigned)n), dim3(64), 0, s, *m, n, C, T, P, cdot); break;
     case 40: hipLaunchKernelGGL(prodrates_kernel<40>, dim3((unsigned)n), dim3(64), 0, s, *m, n, C, T, P, cdot); break;
+    default: return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+extern "C" int tgr_launch_rhs_lanes(int neq, int64_t n, double p_gp, const double* y, double* f, hipStream_t s) {
+  if (n <= 0) return 0;
+  const dim3 grid((unsigned)((n + 63) / 64));
+  switch (neq) {
+    case 10: hipLaunchKernelGGL(rhs_lanes_kernel<10>, grid, dim3(64), 0, s, n, p_gp, y, f); break;
+    case 37: hipLaunchKernelGGL(rhs_lanes_kernel<37>, grid, dim3(64), 0, s, n, p_gp, y, f); break;
+    case 40: hipLaunchKernelGGL(rhs_lanes_kernel<40>, grid, dim3(64), 0, s, n, p_gp, y, f); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;

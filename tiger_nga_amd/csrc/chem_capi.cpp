@@ -69,6 +69,7 @@ struct tgr_chem {
   int nsp = 0, neq = 0;
   double rtol = 1e-6, atol = 1e-12;
   int max_steps = 10000, qmax = 5;
+  int rhs_lanes = 1;  // batched right-hand side: one state per lane (1) or per wave (0)
   int64_t grid_override = 0;
   int device = 0;
   int n_cu = 0;
@@ -286,7 +287,10 @@ int tgr_chem_molar_mass(const tgr_chem* c, double* W) {
 int tgr_chem_set_option(tgr_chem* c, const char* key, int64_t v) {
   if (!c || !key) return fail(TGR_E_ARG, "NULL argument");
   if (!strcmp(key, "grid")) c->grid_override = v;
-  else if (!strcmp(key, "max_order")) {
+  else if (!strcmp(key, "rhs_mapping")) {
+    if (v != 0 && v != 1) return fail(TGR_E_ARG, "rhs_mapping must be 0 (wave per state) or 1 (lane per state)");
+    c->rhs_lanes = (int)v;
+  } else if (!strcmp(key, "max_order")) {
     if (v < 1 || v > 5) return fail(TGR_E_ARG, "max_order must be 1..5");
     c->qmax = (int)v;
   } else if (!strcmp(key, "max_steps")) {
@@ -358,7 +362,9 @@ int tgr_chem_prodrates_device(tgr_chem* c, int64_t n, const double* C, const dou
 int tgr_chem_rhs_device(tgr_chem* c, int64_t n, double p_gp, const double* sol, double* rhs, void* stream) {
   if (!c || (n > 0 && (!sol || !rhs))) return fail(TGR_E_ARG, "NULL argument");
   if (n < 0 || n > 0x7fffffff) return fail(TGR_E_ARG, "bad n");
-  if (tgr_launch_rhs(&c->dm, n, p_gp, sol, rhs, (hipStream_t)stream)) return fail(TGR_E_HIP, "launch");
+  const int rc = c->rhs_lanes ? tgr_launch_rhs_lanes(c->neq, n, p_gp, sol, rhs, (hipStream_t)stream)
+                              : tgr_launch_rhs(&c->dm, n, p_gp, sol, rhs, (hipStream_t)stream);
+  if (rc) return fail(TGR_E_HIP, "launch");
   return 0;
 }
 

@@ -164,15 +164,19 @@ __device__ __forceinline__ double wave_max(double v) {
 // are issued where they are used: without it the compiler hoists every
 // loop-invariant table load out of the solver loops and keeps hundreds of
 // values live in registers across the whole integration.
+// The result is qualified as device memory so that the loads are global
+// (a laundered generic pointer would give flat loads).
 template <class T>
-__device__ __forceinline__ T* fresh(T* p) {
+__device__ __forceinline__ TGR_GLOBAL T* fresh(T* p) {
   asm volatile("" : TGR_OPAQUE_PTR(p));
-  return p;
+  return (TGR_GLOBAL T*)p;
 }
 
+using gchar = const TGR_GLOBAL char;
+
 template <class T>
-__device__ __forceinline__ T ld(const char* base, size_t off) {
-  return *reinterpret_cast<const T*>(base + off);
+__device__ __forceinline__ T ld(gchar* base, size_t off) {
+  return *reinterpret_cast<const TGR_GLOBAL T*>(base + off);
 }
 
 __device__ __forceinline__ double arrhenius(double A, double b, double e, double lt, double irt) {
@@ -182,12 +186,12 @@ __device__ __forceinline__ double arrhenius(double A, double b, double e, double
 // NASA-7 enthalpy / heat capacity of the species in this lane
 // (COMPTHERMODATA, HYDROGENF.f90:514-673); lanes >= nsp return 0.
 template <int NEQ>
-__device__ __forceinline__ void thermo_lane(const char* tab, double tmid, double T, int lane, double& h, double& cp) {
+__device__ __forceinline__ void thermo_lane(gchar* tab, double tmid, double T, int lane, double& h, double& cp) {
   constexpr TabLayout L = tab_layout<NEQ>();
   h = 0.0;
   cp = 0.0;
   if (lane < NEQ - 1) {
-    const char* c = tab + L.th + (T > tmid ? 0 : 13 * 64 * 8) + lane * 8;
+    gchar* c = tab + L.th + (T > tmid ? 0 : 13 * 64 * 8) + lane * 8;
     const double a = ld<double>(c, 0), c1 = ld<double>(c, 512), c2 = ld<double>(c, 1024),
                  c3 = ld<double>(c, 1536), c4 = ld<double>(c, 2048), c5 = ld<double>(c, 2560),
                  c6 = ld<double>(c, 3072);
@@ -218,7 +222,7 @@ __device__ __forceinline__ double prodrates_wave(const DevMech& m, RhsLds<NEQ>& 
   using MT = MechT<NEQ>;
   constexpr TabLayout L = tab_layout<NEQ>();
   const int lane = lane_id();
-  const char* tab = fresh(m.tab);
+  gchar* tab = fresh(m.tab);
   const double lt = flog(T);
   const double rt = kRgasF4 * T;
   const double irt = 1.0 / rt;
@@ -239,13 +243,13 @@ __device__ __forceinline__ double prodrates_wave(const DevMech& m, RhsLds<NEQ>& 
   for (int rd = 0; rd < L.rounds; ++rd) {
     const int r = rd * 64 + lane;
     if (r < MT::nrxn) {
-      const char* rx = tab + L.rx + (size_t)r * 64;
+      gchar* rx = tab + L.rx + (size_t)r * 64;
       const int w = ld<int>(rx, 48), rs = ld<int>(rx, 52), ps = ld<int>(rx, 56);
       const int kind = rm_kind(w), rev = rm_rev(w), nr = rm_nr(w), np = rm_np(w), mi = rm_m(w);
       double kf = arrhenius(ld<double>(rx, 0), ld<double>(rx, 8), ld<double>(rx, 16), lt, irt);
       double kb = rev ? arrhenius(ld<double>(rx, 24), ld<double>(rx, 32), ld<double>(rx, 40), lt, irt) : 0.0;
       if (rd == 0 && kind == 2) {
-        const char* fo = tab + L.fo + (size_t)rm_fo(w) * 128;
+        gchar* fo = tab + L.fo + (size_t)rm_fo(w) * 128;
         const int nfc = ld<int>(fo, 96), kinds = ld<int>(fo, 100);
         double fc = 0.0;
 #pragma unroll
@@ -308,7 +312,7 @@ __device__ __forceinline__ double rhs_wave(const DevMech& m, RhsLds<NEQ>& S, dou
   constexpr TabLayout L = tab_layout<NEQ>();
   const int lane = lane_id();
   const bool sp = lane < nsp;
-  const char* tab = fresh(m.tab);
+  gchar* tab = fresh(m.tab);
   const double rho = wave_sum(sp ? y : 0.0);
   const double T = bcast(y, nsp) / rho;
   double h, cp;
@@ -320,6 +324,21 @@ __device__ __forceinline__ double rhs_wave(const DevMech& m, RhsLds<NEQ>& S, dou
   const double rk = sp ? cdot * ld<double>(tab, L.w + lane * 8) : 0.0;
   const double tmp = wave_sum(-h * rk);
   return sp ? rk : (lane == nsp ? tmp / cpm : 0.0);
+}
+
+// Out-of-line right-hand side for the integrator: one copy of the code
+// serves every call site (the solver calls it from six places), which keeps
+// the integrator's instruction footprint within the instruction cache.  Its
+// LDS scratch is allocated here, once per workgroup.
+template <int NEQ>
+__device__ __attribute__((noinline)) double rhs_call(const char* tab, double tmid, double y, double p_gp) {
+  __shared__ RhsLds<NEQ> S;
+  // arguments arrive in vector registers; the table base is wave-uniform
+  const uint64_t a = (uint64_t)(uintptr_t)tab;
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a >> 32));
+  const DevMech m{(const char*)(uintptr_t)(((uint64_t)hi << 32) | lo), NEQ - 1, tmid};
+  return rhs_wave<NEQ>(m, S, y, p_gp);
 }
 
 }  // namespace tgr

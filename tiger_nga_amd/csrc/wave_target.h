@@ -5,7 +5,11 @@
 #ifdef TGR_WAVE_EMU
 #include "wave_emu.h"
 #define TGR_OPAQUE_PTR "+r"
+#define TGR_GLOBAL
 #else
 #include <hip/hip_runtime.h>
 #define TGR_OPAQUE_PTR "+s"
+// pointee qualifier of device (HBM) memory: loads through it are global_load,
+// not flat_load
+#define TGR_GLOBAL __attribute__((address_space(1)))
 #endif
