@@ -106,7 +106,8 @@ int tgr_chem_prodrates_device(tgr_chem *ctx, int64_t n, const double *C, const d
 int tgr_chem_rhs_device(tgr_chem *ctx, int64_t n, double p_gp, const double *sol, double *rhs, void *stream);
 
 /* Tuning knobs (no counterpart in the reference):
- *   "grid"        size of the persistent grid in blocks (0: occupancy x CUs); blocks pull cells
+ *   "grid"        size of the persistent grid in workgroups of 8 waves (0: occupancy x CUs);
+ *                 every wave pulls cells
  *                 from a device work counter
  *   "rhs_mapping" tgr_chem_rhs_device: 1 (default) one state per lane (generated
  *                 straight-line code of the mechanism), 0 one state per wavefront

@@ -39,6 +39,7 @@ template <int NEQ>
 struct Run {
   const tgr::StepParams* p;
   int* counters;
+  tgr::RhsLds<NEQ> rl;  // shared by the lanes, as the wave's LDS is
   // per-lane copy of the LDS history block: every lane reads and writes only
   // its own column, plus one wave-uniform row that all lanes update with the
   // same values -- one update in SIMT, but sequential fibers sharing it would
@@ -52,7 +53,7 @@ struct Run {
     Run* r = (Run*)(((uintptr_t)hi << 32) | (uintptr_t)lo);
     for (int64_t c = 0; c < r->p->ncell; ++c) {
       int cnt[TGR_NCOUNT] = {0};
-      const int done = tgr::process_cell<NEQ>(*r->p, r->p->m, c, r->zn[l], r->ws.data(), cnt);
+      const int done = tgr::process_cell<NEQ>(*r->p, r->p->m, r->p->m.tab, &r->rl, c, r->zn[l], r->ws.data(), cnt);
       if (l == 0 && r->counters)
         for (int q = 0; q < TGR_NCOUNT; ++q) r->counters[c * TGR_NCOUNT + q] = done ? cnt[q] : 0;
     }

@@ -58,12 +58,14 @@ __device__ __forceinline__ void put(T (&a)[N], int i, T v) {
 template <int NEQ>
 struct Bdf {
   const DevMech m;  // by value: fields stay in scalar registers
+  tchar* tab;        // mechanism tables in LDS
+  TGR_LDS RhsLds<NEQ>* S;  // this wave's right-hand-side scratch in LDS
   double p_gp, rtol, atol;
   int lane, n, qmax, max_steps;
   bool act;
   // per-lane vectors; the Nordsieck history lives in LDS (lane-private
   // columns), so that zn[q] with a run-time order q is a plain address
-  double (&zn)[kQmaxAbs + 2][kWave];
+  TGR_LDS double (&zn)[kQmaxAbs + 2][kWave];
   double ewt, acor, y, tempv, ftemp;
   double* jw;  // saved Jacobian: jw[c * 64 + lane] = J(lane, c)
   double M[NEQ];
@@ -73,15 +75,15 @@ struct Bdf {
   double tn, h, hprime, hscale, eta, etamax, rl1, gamma, gammap, gamrat, crate, acnrm, saved_tq5;
   // small wave-uniform arrays indexed by the run-time order (LDS, every lane
   // writes the same value)
-  double* tq;
-  double* l;
-  double* tau;
+  TGR_LDS double* tq;
+  TGR_LDS double* l;
+  TGR_LDS double* tau;
   int q, qprime, qwait, Lq, jcur, nst, nstlp, nstlj;
   int c_nfe, c_nfe_dq, c_nje, c_nsetups, c_nni, c_netf, c_ncfn, c_nlu;
 
-  __device__ Bdf(const DevMech& mech, double (&znl)[kQmaxAbs + 2][kWave], double* ws, double pgp,
-                 double rt, double at, int qm, int ms)
-      : m(mech), p_gp(pgp), rtol(rt), atol(at), max_steps(ms), zn(znl), jw(ws) {
+  __device__ Bdf(const DevMech& mech, tchar* tabl, TGR_LDS RhsLds<NEQ>* rl, TGR_LDS double (&znl)[kQmaxAbs + 2][kWave],
+                 double* ws, double pgp, double rt, double at, int qm, int ms)
+      : m(mech), tab(tabl), S(rl), p_gp(pgp), rtol(rt), atol(at), max_steps(ms), zn(znl), jw(ws) {
     tq = &znl[kQmaxAbs + 1][0];
     l = tq + 8;
     tau = tq + 16;
@@ -91,7 +93,7 @@ struct Bdf {
     qmax = (qm >= 1 && qm <= kQmaxAbs) ? qm : kQmaxAbs;
   }
 
-  __device__ __forceinline__ double f(double yv) { return rhs_call<NEQ>(m.tab, m.tmid, act ? yv : 0.0, p_gp); }
+  __device__ __forceinline__ double f(double yv) { return rhs_call<NEQ>(tab, S, m.tmid, act ? yv : 0.0, p_gp); }
 
   __device__ __forceinline__ double wrms(double x) {
     double v = act ? x * ewt : 0.0;
@@ -663,7 +665,9 @@ struct Bdf {
 // RHO_, heat release and H2O/CO/CO2 sources.  Returns 0 for a wall cell
 // (mask == 1, skipped), else 1 with the integrator counters in cnt.
 template <int NEQ>
-__device__ __forceinline__ int process_cell(const StepParams& p, const DevMech& m, int64_t cell, double (&zn_lds)[kQmaxAbs + 2][kWave], double* ws, int* cnt) {
+__device__ __forceinline__ int process_cell(const StepParams& p, const DevMech& m, tchar* tab, TGR_LDS RhsLds<NEQ>* rl,
+                                            int64_t cell, TGR_LDS double (&zn_lds)[kQmaxAbs + 2][kWave], double* ws,
+                                            int* cnt) {
   const int lane = lane_id();
   const int nsp = NEQ - 1;
   const int i = (int)(cell % p.nx);
@@ -679,7 +683,7 @@ __device__ __forceinline__ int process_cell(const StepParams& p, const DevMech& 
     if (lane < nsp) sol = fmax(rho * p.sc[(int64_t)(p.isc_sc0 + lane) * p.plane + off], 0.0);
     else if (lane == nsp) sol = rho * p.sc[(int64_t)p.isc_T0 * p.plane + off];
     const double sol0 = sol;
-    Bdf<NEQ> b(m, zn_lds, ws, p.p_gp, p.rtol, p.atol, p.qmax, p.max_steps);
+    Bdf<NEQ> b(m, tab, rl, zn_lds, ws, p.p_gp, p.rtol, p.atol, p.qmax, p.max_steps);
     const int st = b.integrate(p.dt, sol);
     // unpack (finitechem.f90:1041-1042) and diagnostics (1047-1058)
     const double v = sol / rho;
@@ -687,7 +691,7 @@ __device__ __forceinline__ int process_cell(const StepParams& p, const DevMech& 
     else if (lane == nsp) p.sc[(int64_t)p.isc_T0 * p.plane + off] = v;
     const double Tn = bcast(v, nsp);
     double h, cp;
-    thermo_lane<NEQ>(fresh(m.tab), m.tmid, Tn, lane, h, cp);
+    thermo_lane<NEQ>(tab, m.tmid, Tn, lane, h, cp);
     const double cpm = (wave_sum(lane < nsp ? v * cp : 0.0));
     if (p.hr && lane == nsp) p.hr[off] = cpm * (sol - sol0) / p.dt;
     if (p.chemsrc) {

@@ -1,5 +1,9 @@
 // HIP kernels of the chemistry step (gfx950): the persistent step kernel,
 // the component kernels used by the parity tests, and their launchers.
+//
+// Every kernel runs kWavesPerBlock independent wavefronts per workgroup; the
+// workgroup copies the mechanism tables into LDS once, and each wave owns its
+// right-hand-side scratch and Nordsieck history in LDS.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -11,22 +15,36 @@
 
 namespace tgr {
 
-// finitechem_source_chemistry on a Fortran-layout block.  One wavefront per
-// workgroup; workgroups pull cells from a device counter until all are taken,
-// so that cells whose step counts differ by 10-100x keep every wave busy.
+// 8 waves = 2 per SIMD: the integrator needs up to 256 VGPRs per lane.
+constexpr int kWavesPerBlock = 8;
+constexpr int kBlock = 64 * kWavesPerBlock;
+
+__device__ __forceinline__ int wave_in_block() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
+// finitechem_source_chemistry on a Fortran-layout block.  Persistent: each
+// wave pulls cells from a device counter until all are taken, so that cells
+// whose step counts differ by 10-100x keep every wave busy.
 template <int NEQ>
-__global__ __launch_bounds__(64, 2) void step_kernel(StepParams p) {
-  __shared__ double zn_lds[kQmaxAbs + 2][kWave];  // Nordsieck columns + tq, l, tau
+__global__ __launch_bounds__(kBlock) void step_kernel(StepParams p) {
+  constexpr TabLayout L = tab_layout<NEQ>();
+  __shared__ __attribute__((aligned(16))) char tab_s[L.bytes];
+  __shared__ RhsLds<NEQ> rl[kWavesPerBlock];
+  __shared__ double zn_s[kWavesPerBlock][kQmaxAbs + 2][kWave];  // Nordsieck columns + tq, l, tau
+  stage_tables((TGR_LDS char*)tab_s, p.m.tab, L.bytes);
+  __syncthreads();
   const DevMech m = p.m;
   const int lane = lane_id();
-  double* ws = p.ws + (int64_t)blockIdx.x * NEQ * kWave;
+  const int w = wave_in_block();
+  double* ws = p.ws + ((int64_t)blockIdx.x * kWavesPerBlock + w) * NEQ * kWave;
   for (;;) {
     unsigned int c = 0;
     if (lane == 0) c = atomicAdd(p.queue, 1u);
     const int64_t cell = (int64_t)(unsigned int)__builtin_amdgcn_readfirstlane((int)c);
     if (cell >= p.ncell) break;
     int cnt[TGR_NCOUNT];
-    if (!process_cell<NEQ>(p, m, cell, zn_lds, ws, cnt)) continue;
+    if (!process_cell<NEQ>(p, m, (tchar*)tab_s, (TGR_LDS RhsLds<NEQ>*)&rl[w], cell,
+                           *(TGR_LDS double(*)[kQmaxAbs + 2][kWave]) & zn_s[w], ws, cnt))
+      continue;
     if (lane == 0) {
       if (p.cell_stats) {
 #pragma unroll
@@ -42,30 +60,41 @@ __global__ __launch_bounds__(64, 2) void step_kernel(StepParams p) {
 }
 
 // ---------------------------------------------------------------------------
-// component kernels (one wave per state): production rates and RHS
+// component kernels (one wave per state, grid-stride over states):
+// production rates and the right-hand side
 // ---------------------------------------------------------------------------
 template <int NEQ>
-__global__ __launch_bounds__(64) void prodrates_kernel(DevMech m, int64_t n, const double* C, const double* T,
-                                                      const double* P, double* cdot) {
-  __shared__ RhsLds<NEQ> lds;
+__global__ __launch_bounds__(kBlock) void prodrates_kernel(DevMech m, int64_t n, const double* C, const double* T,
+                                                           const double* P, double* cdot) {
+  constexpr TabLayout L = tab_layout<NEQ>();
+  __shared__ __attribute__((aligned(16))) char tab_s[L.bytes];
+  __shared__ RhsLds<NEQ> rl[kWavesPerBlock];
+  stage_tables((TGR_LDS char*)tab_s, m.tab, L.bytes);
+  __syncthreads();
   const int lane = lane_id();
-  const int64_t c = blockIdx.x;
-  if (c >= n) return;
-  const double Ck = lane < m.nsp ? C[lane * n + c] : 0.0;
-  const double ctot = (wave_sum(Ck));
-  const double v = prodrates_wave<NEQ>(m, lds, Ck, ctot, T[c], P[c]);
-  if (lane < m.nsp) cdot[lane * n + c] = v;
+  const int w = wave_in_block();
+  for (int64_t c = (int64_t)blockIdx.x * kWavesPerBlock + w; c < n; c += (int64_t)gridDim.x * kWavesPerBlock) {
+    const double Ck = lane < m.nsp ? C[lane * n + c] : 0.0;
+    const double ctot = wave_sum(Ck);
+    const double v = prodrates_wave<NEQ>((tchar*)tab_s, *(TGR_LDS RhsLds<NEQ>*)&rl[w], Ck, ctot, T[c], P[c]);
+    if (lane < m.nsp) cdot[lane * n + c] = v;
+  }
 }
 
 template <int NEQ>
-__global__ __launch_bounds__(64) void rhs_kernel(DevMech m, int64_t n, double p_gp, const double* y, double* f) {
-  __shared__ RhsLds<NEQ> lds;
+__global__ __launch_bounds__(kBlock) void rhs_kernel(DevMech m, int64_t n, double p_gp, const double* y, double* f) {
+  constexpr TabLayout L = tab_layout<NEQ>();
+  __shared__ __attribute__((aligned(16))) char tab_s[L.bytes];
+  __shared__ RhsLds<NEQ> rl[kWavesPerBlock];
+  stage_tables((TGR_LDS char*)tab_s, m.tab, L.bytes);
+  __syncthreads();
   const int lane = lane_id();
-  const int64_t c = blockIdx.x;
-  if (c >= n) return;
-  const double yv = lane < NEQ ? y[lane * n + c] : 0.0;
-  const double v = rhs_wave<NEQ>(m, lds, yv, p_gp);
-  if (lane < NEQ) f[lane * n + c] = v;
+  const int w = wave_in_block();
+  for (int64_t c = (int64_t)blockIdx.x * kWavesPerBlock + w; c < n; c += (int64_t)gridDim.x * kWavesPerBlock) {
+    const double yv = lane < NEQ ? y[lane * n + c] : 0.0;
+    const double v = rhs_wave<NEQ>((tchar*)tab_s, *(TGR_LDS RhsLds<NEQ>*)&rl[w], m.tmid, yv, p_gp);
+    if (lane < NEQ) f[lane * n + c] = v;
+  }
 }
 
 // Lane per state: each lane evaluates a whole right-hand side with the
@@ -85,17 +114,26 @@ __global__ __launch_bounds__(64) void rhs_lanes_kernel(int64_t n, double p_gp, c
 
 __global__ void reset_counter_kernel(unsigned int* q) { *q = 0u; }
 
+// grid of the component kernels: enough workgroups to fill the GPU, each
+// looping over states (the tables are staged once per workgroup)
+inline unsigned component_grid(int64_t n) {
+  const int64_t need = (n + kWavesPerBlock - 1) / kWavesPerBlock;
+  return (unsigned)(need < 2048 ? need : 2048);
+}
+
 }  // namespace tgr
 
 using namespace tgr;
+
+extern "C" int tgr_step_block_waves(void) { return kWavesPerBlock; }
 
 extern "C" int tgr_step_occupancy(int neq) {
   int nb = 0;
   hipError_t e = hipErrorInvalidValue;
   switch (neq) {
-    case 10: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_kernel<10>, 64, 0); break;
-    case 37: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_kernel<37>, 64, 0); break;
-    case 40: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_kernel<40>, 64, 0); break;
+    case 10: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_kernel<10>, kBlock, 0); break;
+    case 37: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_kernel<37>, kBlock, 0); break;
+    case 40: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_kernel<40>, kBlock, 0); break;
     default: break;
   }
   return e == hipSuccess ? nb : 0;
@@ -105,9 +143,9 @@ extern "C" int tgr_launch_step(const StepParams* p, int neq, int grid, hipStream
   if (!p->queue || !p->ws) return -1;
   hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, p->queue);
   switch (neq) {
-    case 10: hipLaunchKernelGGL(step_kernel<10>, dim3(grid), dim3(64), 0, s, *p); break;
-    case 37: hipLaunchKernelGGL(step_kernel<37>, dim3(grid), dim3(64), 0, s, *p); break;
-    case 40: hipLaunchKernelGGL(step_kernel<40>, dim3(grid), dim3(64), 0, s, *p); break;
+    case 10: hipLaunchKernelGGL(step_kernel<10>, dim3(grid), dim3(kBlock), 0, s, *p); break;
+    case 37: hipLaunchKernelGGL(step_kernel<37>, dim3(grid), dim3(kBlock), 0, s, *p); break;
+    case 40: hipLaunchKernelGGL(step_kernel<40>, dim3(grid), dim3(kBlock), 0, s, *p); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
@@ -116,10 +154,11 @@ extern "C" int tgr_launch_step(const StepParams* p, int neq, int grid, hipStream
 extern "C" int tgr_launch_prodrates(const DevMech* m, int64_t n, const double* C, const double* T, const double* P,
                                     double* cdot, hipStream_t s) {
   if (n <= 0) return 0;
+  const dim3 g(component_grid(n)), b(kBlock);
   switch (m->nsp + 1) {
-    case 10: hipLaunchKernelGGL(prodrates_kernel<10>, dim3((unsigned)n), dim3(64), 0, s, *m, n, C, T, P, cdot); break;
-    case 37: hipLaunchKernelGGL(prodrates_kernel<37>, dim3((unsigned)n), dim3(64), 0, s, *m, n, C, T, P, cdot); break;
-    case 40: hipLaunchKernelGGL(prodrates_kernel<40>, dim3((unsigned)n), dim3(64), 0, s, *m, n, C, T, P, cdot); break;
+    case 10: hipLaunchKernelGGL(prodrates_kernel<10>, g, b, 0, s, *m, n, C, T, P, cdot); break;
+    case 37: hipLaunchKernelGGL(prodrates_kernel<37>, g, b, 0, s, *m, n, C, T, P, cdot); break;
+    case 40: hipLaunchKernelGGL(prodrates_kernel<40>, g, b, 0, s, *m, n, C, T, P, cdot); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
@@ -139,10 +178,11 @@ extern "C" int tgr_launch_rhs_lanes(int neq, int64_t n, double p_gp, const doubl
 
 extern "C" int tgr_launch_rhs(const DevMech* m, int64_t n, double p_gp, const double* y, double* f, hipStream_t s) {
   if (n <= 0) return 0;
+  const dim3 g(component_grid(n)), b(kBlock);
   switch (m->nsp + 1) {
-    case 10: hipLaunchKernelGGL(rhs_kernel<10>, dim3((unsigned)n), dim3(64), 0, s, *m, n, p_gp, y, f); break;
-    case 37: hipLaunchKernelGGL(rhs_kernel<37>, dim3((unsigned)n), dim3(64), 0, s, *m, n, p_gp, y, f); break;
-    case 40: hipLaunchKernelGGL(rhs_kernel<40>, dim3((unsigned)n), dim3(64), 0, s, *m, n, p_gp, y, f); break;
+    case 10: hipLaunchKernelGGL(rhs_kernel<10>, g, b, 0, s, *m, n, p_gp, y, f); break;
+    case 37: hipLaunchKernelGGL(rhs_kernel<37>, g, b, 0, s, *m, n, p_gp, y, f); break;
+    case 40: hipLaunchKernelGGL(rhs_kernel<40>, g, b, 0, s, *m, n, p_gp, y, f); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;

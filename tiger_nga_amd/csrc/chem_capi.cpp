@@ -157,14 +157,16 @@ int run_step(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const
   p.ncell = (int64_t)nx * ny * nz;
   p.totals = c->d_totals;
   p.cell_stats = nullptr;
-  int per_cu = 8;
+  int per_cu = 1;
   if (c->grid_override <= 0) {
-    int nb = tgr_step_occupancy(c->neq);
+    const int nb = tgr_step_occupancy(c->neq);
     if (nb > 0) per_cu = nb;
   }
+  // persistent grid: every wave of every resident workgroup pulls cells
+  const int wpb = tgr_step_block_waves();
   const int64_t gs = c->grid_override > 0 ? c->grid_override : (int64_t)c->n_cu * per_cu;
-  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(gs, p.ncell));
-  int rc0 = c->b_ws.ensure((size_t)grid * c->neq * 64 * sizeof(double));
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(gs, (p.ncell + wpb - 1) / wpb));
+  int rc0 = c->b_ws.ensure((size_t)grid * wpb * c->neq * 64 * sizeof(double));
   if (rc0) return rc0;
   p.queue = c->d_queue;
   p.ws = (double*)c->b_ws.p;

@@ -11,7 +11,7 @@
 //   * reactions are dealt to lanes round-robin (fall-off reactions first so
 //     that only the first round diverges), each lane computing k_f, k_b
 //     (Arrhenius / GETLINDRATECOEFF Troe form, HYDROGENF.f90:280-316) and
-//     the net rate of progress q = W_f - W_b into LDS;
+//     the net rate of progress q = (W_f - W_b) [* M] into LDS;
 //   * CDOT is gathered deterministically: the (reaction, nu) entries of each
 //     species are split into lane-sized chunks, summed per lane, then summed
 //     per species (fixed order, no atomics).
@@ -19,10 +19,14 @@
 // (combustion/finitechem.f90:93-175): rho = sum(rho*Y), T = rhoT/rho,
 // C = rho*Y/W, rhs_k = CDOT_k*W_k, rhs_T = -sum(h_k rhs_k)/cp_mix.
 //
-// Tables live in one device allocation whose layout is a compile-time
-// function of the mechanism shape (MechT), so that every table load is a
-// single base register plus lane offset plus immediate: nothing per-table has
-// to stay live in registers across the solver loop.
+// The tables are one block whose layout is a compile-time function of the
+// mechanism shape (MechT), structure-of-arrays with one 64-lane row per
+// field, so that every access of a wave is one conflict-free LDS read.  The
+// kernels copy the block into LDS once per workgroup.  Records are padded so
+// that the reaction loop has no data-dependent branches except the fall-off
+// form: unused concentration factors and the third-body factor of reactions
+// without one point to an LDS slot holding 1.0, irreversible reactions carry
+// A = 0, and padding entries of the gather and correction lists have weight 0.
 #pragma once
 #include "wave_target.h"
 #include <stdint.h>
@@ -33,6 +37,9 @@ namespace tgr {
 
 constexpr int kWave = 64;
 constexpr int kMaxNeq = 64;
+// LDS slots of the concentration / third-body vectors holding 1.0 and 0.0
+constexpr int kOne = 63;
+constexpr int kZero = 62;
 
 // binary32 values of the reference's unsuffixed literals (Makefile.in:19 sets
 // no real promotion): R = 8314.34 and the Troe constants of GETLINDRATECOEFF.
@@ -63,22 +70,30 @@ struct MechT<40> {  // dme39 (DME39F)
   static constexpr int nrxn = 175, nfo = 11, nthird = 20, gmax = 15, maxe = 6;
 };
 
-// Byte layout of the table block for a mechanism shape.
-//   RX  [rounds*64][8] doubles: A_f b_f e_f A_b b_b e_b | meta, reac, prod, 0 (int32)
-//   FO  [64][16] doubles: A0f b0f e0f A0b b0b e0b c0 c1 c2 d0 d1 d2 | nfc, kinds, 0, 0 (int32) | pad
-//   TH  [2][13][64] doubles (coefficient-major, species in lanes)
-//   W, IW [64] doubles
-//   EV  [maxe][64] doubles, EK [maxe][64] int32, EN [64] int32
-//   GN  [gmax][64] doubles, GR [gmax][64] int32, GC, SF, SN [64] int32
+// Byte layout of the table block for a mechanism shape (rows of 64 lanes:
+// a double row is 512 bytes, an int row 256).
+//   RX  per 64-reaction round: double rows A_f b_f e_f A_b b_b e_b, int rows
+//       I0 = r0 | r1 << 8 | r2 << 16 | m << 24 (reactant slots, third-body slot)
+//       I1 = p0 | p1 << 8 | p2 << 16 | falloff << 24 | rev << 25
+//   FO  (round-0 lanes < nfo) double rows A0_f b0_f e0_f A0_b b0_b e0_b c0 c1 c2
+//       g0 g1 g2 and an int row kinds | mfo << 8: Fc = sum_i c_i exp(-g_i T) or,
+//       where bit i of kinds is set, c_i exp(-g_i / T); mfo: slot of the
+//       third-body sum of GETLINDRATECOEFF (kZero: none, conc = P/RT)
+//   TH  [2][13] double rows (range, coefficient), species in lanes
+//   W, IW  double rows: molar mass and its reciprocal
+//   EV, EK  [maxe] double / int rows: third-body corrections (eff - 1, slot)
+//   GN, GR  [gmax] double / int rows: CDOT gather (nu, q index)
+//   SF, SN  int rows: first gather lane and number of chunks of each species
+constexpr size_t kRow = 512, kIRow = 256;
+constexpr size_t kRxRound = 6 * kRow + 2 * kIRow;
 struct TabLayout {
   int nsp, nrxn, rounds, nfo, nthird, gmax, maxe;
-  size_t rx, fo, th, w, iw, ev, ek, en, gn, gr, gc, sf, sn, bytes;
+  size_t rx, fo, th, w, iw, ev, ek, gn, gr, sf, sn, bytes;
   __host__ __device__ constexpr TabLayout(int nsp_, int nrxn_, int nfo_, int nthird_, int gmax_, int maxe_)
       : nsp(nsp_), nrxn(nrxn_), rounds((nrxn_ + 63) / 64), nfo(nfo_), nthird(nthird_), gmax(gmax_), maxe(maxe_),
-        rx(0), fo(rx + (size_t)((nrxn_ + 63) / 64) * 64 * 64), th(fo + 64 * 128), w(th + 2 * 13 * 64 * 8),
-        iw(w + 64 * 8), ev(iw + 64 * 8), ek(ev + (size_t)maxe_ * 64 * 8), en(ek + (size_t)maxe_ * 64 * 4),
-        gn(en + 64 * 4), gr(gn + (size_t)gmax_ * 64 * 8), gc(gr + (size_t)gmax_ * 64 * 4), sf(gc + 64 * 4),
-        sn(sf + 64 * 4), bytes(sn + 64 * 4) {}
+        rx(0), fo(rx + (size_t)((nrxn_ + 63) / 64) * kRxRound), th(fo + 12 * kRow + kIRow), w(th + 26 * kRow),
+        iw(w + kRow), ev(iw + kRow), ek(ev + (size_t)maxe_ * kRow), gn(ek + (size_t)maxe_ * kIRow),
+        gr(gn + (size_t)gmax_ * kRow), sf(gr + (size_t)gmax_ * kIRow), sn(sf + kIRow), bytes(sn + kIRow) {}
 };
 
 template <int NEQ>
@@ -87,20 +102,15 @@ __host__ __device__ constexpr TabLayout tab_layout() {
                    MechT<NEQ>::maxe);
 }
 
-// meta word: kind (2 bits) | rev (1) | nr (2) | np (2) | m+1 (7) | fo index (8)
-__host__ __device__ inline int rm_kind(int w) { return w & 3; }
-__host__ __device__ inline int rm_rev(int w) { return (w >> 2) & 1; }
-__host__ __device__ inline int rm_nr(int w) { return (w >> 3) & 3; }
-__host__ __device__ inline int rm_np(int w) { return (w >> 5) & 3; }
-__host__ __device__ inline int rm_m(int w) { return ((w >> 7) & 127) - 1; }
-__host__ __device__ inline int rm_fo(int w) { return (w >> 14) & 255; }
-
-// Device view of a mechanism: the table block plus a few sizes.
+// Device view of a mechanism: the table block (HBM) plus a few sizes.
 struct DevMech {
   const char* tab;
   int nsp;
   double tmid;
 };
+
+// Table block as the kernels read it (LDS).
+using tchar = const TGR_LDS char;
 
 // Per-wave LDS scratch for one right-hand-side evaluation.
 template <int NEQ>
@@ -160,24 +170,36 @@ __device__ __forceinline__ double wave_max(double v) {
   return bcast(s, 63);
 }
 
-// Returns the table base through an empty asm statement so that table loads
-// are issued where they are used: without it the compiler hoists every
-// loop-invariant table load out of the solver loops and keeps hundreds of
-// values live in registers across the whole integration.
-// The result is qualified as device memory so that the loads are global
-// (a laundered generic pointer would give flat loads).
+// Returns the pointer through an empty asm statement so that loads through
+// it are issued where they are used: without it the compiler hoists every
+// loop-invariant load out of the solver loops and keeps hundreds of values
+// live in registers across the whole integration.  The result is qualified
+// as device memory so that the loads are global (a laundered generic pointer
+// would give flat loads).
 template <class T>
 __device__ __forceinline__ TGR_GLOBAL T* fresh(T* p) {
   asm volatile("" : TGR_OPAQUE_PTR(p));
   return (TGR_GLOBAL T*)p;
 }
 
-using gchar = const TGR_GLOBAL char;
-
 template <class T>
-__device__ __forceinline__ T ld(gchar* base, size_t off) {
-  return *reinterpret_cast<const TGR_GLOBAL T*>(base + off);
+__device__ __forceinline__ T ld(tchar* base, size_t off) {
+  return *reinterpret_cast<const TGR_LDS T*>(base + off);
 }
+
+#ifndef TGR_WAVE_EMU
+// Copies the table block from HBM into LDS (all threads of the workgroup,
+// 16-byte chunks); the caller synchronises the workgroup afterwards.
+__device__ __forceinline__ void stage_tables(TGR_LDS char* dst, const char* src, size_t bytes) {
+  const TGR_GLOBAL uint64_t* s8 = (const TGR_GLOBAL uint64_t*)src;
+  TGR_LDS uint64_t* d8 = (TGR_LDS uint64_t*)dst;
+  for (size_t i = threadIdx.x; i < bytes / 16; i += blockDim.x) {
+    const uint64_t a = s8[2 * i], b = s8[2 * i + 1];
+    d8[2 * i] = a;
+    d8[2 * i + 1] = b;
+  }
+}
+#endif
 
 __device__ __forceinline__ double arrhenius(double A, double b, double e, double lt, double irt) {
   return A * fexp(fma(b, lt, e * irt));
@@ -186,20 +208,17 @@ __device__ __forceinline__ double arrhenius(double A, double b, double e, double
 // NASA-7 enthalpy / heat capacity of the species in this lane
 // (COMPTHERMODATA, HYDROGENF.f90:514-673); lanes >= nsp return 0.
 template <int NEQ>
-__device__ __forceinline__ void thermo_lane(gchar* tab, double tmid, double T, int lane, double& h, double& cp) {
+__device__ __forceinline__ void thermo_lane(tchar* tab, double tmid, double T, int lane, double& h, double& cp) {
   constexpr TabLayout L = tab_layout<NEQ>();
-  h = 0.0;
-  cp = 0.0;
-  if (lane < NEQ - 1) {
-    gchar* c = tab + L.th + (T > tmid ? 0 : 13 * 64 * 8) + lane * 8;
-    const double a = ld<double>(c, 0), c1 = ld<double>(c, 512), c2 = ld<double>(c, 1024),
-                 c3 = ld<double>(c, 1536), c4 = ld<double>(c, 2048), c5 = ld<double>(c, 2560),
-                 c6 = ld<double>(c, 3072);
-    const double ac = ld<double>(c, 3584), d1 = ld<double>(c, 4096), d2 = ld<double>(c, 4608),
-                 d3 = ld<double>(c, 5120), d4 = ld<double>(c, 5632), d5 = ld<double>(c, 6144);
-    h = a * (T * (c1 + T * (c2 + T * (c3 + T * (c4 + T * c5)))) + c6);
-    cp = ac * (d1 + T * (d2 + T * (d3 + T * (d4 + T * d5))));
-  }
+  tchar* c = tab + L.th + (T > tmid ? 0 : 13 * kRow) + lane * 8;
+  const double a = ld<double>(c, 0), c1 = ld<double>(c, kRow), c2 = ld<double>(c, 2 * kRow),
+               c3 = ld<double>(c, 3 * kRow), c4 = ld<double>(c, 4 * kRow), c5 = ld<double>(c, 5 * kRow),
+               c6 = ld<double>(c, 6 * kRow);
+  const double ac = ld<double>(c, 7 * kRow), d1 = ld<double>(c, 8 * kRow), d2 = ld<double>(c, 9 * kRow),
+               d3 = ld<double>(c, 10 * kRow), d4 = ld<double>(c, 11 * kRow), d5 = ld<double>(c, 12 * kRow);
+  h = a * (T * (c1 + T * (c2 + T * (c3 + T * (c4 + T * c5)))) + c6);
+  cp = ac * (d1 + T * (d2 + T * (d3 + T * (d4 + T * d5))));
+  if (lane >= NEQ - 1) h = cp = 0.0;
 }
 
 // Troe fall-off rate (GETLINDRATECOEFF, HYDROGENF.f90:280-316, ITROE = 1):
@@ -217,80 +236,66 @@ __device__ __forceinline__ double troe(double kinf, double k0, double conc, doub
 // concentrations (lane k holds C_k), temperature and pressure.  ctot is the
 // total concentration sum_k C_k (wave-uniform).
 template <int NEQ>
-__device__ __forceinline__ double prodrates_wave(const DevMech& m, RhsLds<NEQ>& S, double Ck, double ctot, double T,
-                                                 double P) {
+__device__ __forceinline__ double prodrates_wave(tchar* tab, TGR_LDS RhsLds<NEQ>& S, double Ck, double ctot,
+                                                 double T, double P) {
   using MT = MechT<NEQ>;
   constexpr TabLayout L = tab_layout<NEQ>();
   const int lane = lane_id();
-  gchar* tab = fresh(m.tab);
   const double lt = flog(T);
   const double rt = kRgasF4 * T;
   const double irt = 1.0 / rt;
-  S.C[lane] = Ck;
+  S.C[lane] = lane == kOne ? 1.0 : Ck;
   wave_sync();
-  // third-body sums: M_j = ctot + sum_e (eff_je - 1) C_e
-  if (lane < MT::nthird) {
+  // third-body sums: M_j = ctot + sum_e (eff_je - 1) C_e (padding: weight 0 on slot kOne)
+  {
     double s = ctot;
-    const int ne = ld<int>(tab, L.en + lane * 4);
 #pragma unroll
     for (int e = 0; e < MT::maxe; ++e)
-      if (e < ne) s = fma(ld<double>(tab, L.ev + (e * 64 + lane) * 8), S.C[ld<int>(tab, L.ek + (e * 64 + lane) * 4)], s);
-    S.M[lane] = s;
+      s = fma(ld<double>(tab, L.ev + e * kRow + lane * 8), S.C[ld<int>(tab, L.ek + e * kIRow + lane * 4)], s);
+    S.M[lane] = lane == kOne ? 1.0 : (lane == kZero ? 0.0 : s);
   }
   wave_sync();
   // rates of progress, one reaction per lane per round (fall-off in round 0)
 #pragma unroll
   for (int rd = 0; rd < L.rounds; ++rd) {
-    const int r = rd * 64 + lane;
-    if (r < MT::nrxn) {
-      gchar* rx = tab + L.rx + (size_t)r * 64;
-      const int w = ld<int>(rx, 48), rs = ld<int>(rx, 52), ps = ld<int>(rx, 56);
-      const int kind = rm_kind(w), rev = rm_rev(w), nr = rm_nr(w), np = rm_np(w), mi = rm_m(w);
-      double kf = arrhenius(ld<double>(rx, 0), ld<double>(rx, 8), ld<double>(rx, 16), lt, irt);
-      double kb = rev ? arrhenius(ld<double>(rx, 24), ld<double>(rx, 32), ld<double>(rx, 40), lt, irt) : 0.0;
-      if (rd == 0 && kind == 2) {
-        gchar* fo = tab + L.fo + (size_t)rm_fo(w) * 128;
-        const int nfc = ld<int>(fo, 96), kinds = ld<int>(fo, 100);
-        double fc = 0.0;
+    tchar* rx = tab + L.rx + rd * kRxRound + lane * 8;
+    tchar* rxi = tab + L.rx + rd * kRxRound + 6 * kRow + lane * 4;
+    const int i0 = ld<int>(rxi, 0), i1 = ld<int>(rxi, kIRow);
+    double kf = arrhenius(ld<double>(rx, 0), ld<double>(rx, kRow), ld<double>(rx, 2 * kRow), lt, irt);
+    double kb = arrhenius(ld<double>(rx, 3 * kRow), ld<double>(rx, 4 * kRow), ld<double>(rx, 5 * kRow), lt, irt);
+    if (rd == 0 && ((i1 >> 24) & 1)) {
+      tchar* fo = tab + L.fo + lane * 8;
+      const int fi = ld<int>(tab, L.fo + 12 * kRow + lane * 4);
+      const double invT = 1.0 / T;
+      double fc = 0.0;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          if (i < nfc) {
-            const double c = ld<double>(fo, 48 + 8 * i), d = ld<double>(fo, 72 + 8 * i);
-            const double x = ((kinds >> i) & 1) ? -(d / T) : -(T / d);
-            fc = fma(c, fexp(x), fc);
-          }
-        }
-        const double conc = (mi >= 0 && S.M[mi] > 0.0) ? S.M[mi] : P / rt;
-        const double lnfc = flog(fc);
-        const double lfc = lnfc * kInvLn10;
-        const double ntmp = fma(-kTroeN1, lfc, kTroeN0);
-        const double cco = fma(-kTroeC1, lfc, -kTroeC0);
-        kf = troe(kf, arrhenius(ld<double>(fo, 0), ld<double>(fo, 8), ld<double>(fo, 16), lt, irt), conc, lnfc,
-                  ntmp, cco);
-        if (rev)
-          kb = troe(kb, arrhenius(ld<double>(fo, 24), ld<double>(fo, 32), ld<double>(fo, 40), lt, irt), conc, lnfc,
-                    ntmp, cco);
+      for (int i = 0; i < 3; ++i) {
+        const double x = -(ld<double>(fo, (9 + i) * kRow) * (((fi >> i) & 1) ? invT : T));
+        fc = fma(ld<double>(fo, (6 + i) * kRow), fexp(x), fc);
       }
-      double wf = kf, wb = kb;
-      if (nr > 0) wf *= S.C[rs & 255];
-      if (nr > 1) wf *= S.C[(rs >> 8) & 255];
-      if (nr > 2) wf *= S.C[(rs >> 16) & 255];
-      if (np > 0) wb *= S.C[ps & 255];
-      if (np > 1) wb *= S.C[(ps >> 8) & 255];
-      if (np > 2) wb *= S.C[(ps >> 16) & 255];
-      double q = wf - wb;
-      if (kind == 1) q *= S.M[mi];
-      S.q[r] = q;
+      const double mv = S.M[(fi >> 8) & 255];
+      const double conc = mv > 0.0 ? mv : P / rt;
+      const double lnfc = flog(fc);
+      const double lfc = lnfc * kInvLn10;
+      const double ntmp = fma(-kTroeN1, lfc, kTroeN0);
+      const double cco = fma(-kTroeC1, lfc, -kTroeC0);
+      kf = troe(kf, arrhenius(ld<double>(fo, 0), ld<double>(fo, kRow), ld<double>(fo, 2 * kRow), lt, irt), conc,
+                lnfc, ntmp, cco);
+      if ((i1 >> 25) & 1)
+        kb = troe(kb, arrhenius(ld<double>(fo, 3 * kRow), ld<double>(fo, 4 * kRow), ld<double>(fo, 5 * kRow), lt, irt),
+                  conc, lnfc, ntmp, cco);
     }
+    const double wf = kf * S.C[i0 & 255] * S.C[(i0 >> 8) & 255] * S.C[(i0 >> 16) & 255];
+    const double wb = kb * S.C[i1 & 255] * S.C[(i1 >> 8) & 255] * S.C[(i1 >> 16) & 255];
+    S.q[rd * 64 + lane] = (wf - wb) * S.M[(i0 >> 24) & 255];
   }
   wave_sync();
   // deterministic gather of CDOT
   {
     double acc = 0.0;
-    const int cnt = ld<int>(tab, L.gc + lane * 4);
 #pragma unroll
     for (int i = 0; i < MT::gmax; ++i)
-      if (i < cnt) acc = fma(ld<double>(tab, L.gn + (i * 64 + lane) * 8), S.q[ld<int>(tab, L.gr + (i * 64 + lane) * 4)], acc);
+      acc = fma(ld<double>(tab, L.gn + i * kRow + lane * 8), S.q[ld<int>(tab, L.gr + i * kIRow + lane * 4)], acc);
     S.part[lane] = acc;
   }
   wave_sync();
@@ -307,20 +312,19 @@ __device__ __forceinline__ double prodrates_wave(const DevMech& m, RhsLds<NEQ>& 
 // y: this lane's component of [rho*Y (nsp), rho*T]; returns the same lane of
 // the right-hand side.
 template <int NEQ>
-__device__ __forceinline__ double rhs_wave(const DevMech& m, RhsLds<NEQ>& S, double y, double p_gp) {
+__device__ __forceinline__ double rhs_wave(tchar* tab, TGR_LDS RhsLds<NEQ>& S, double tmid, double y, double p_gp) {
   constexpr int nsp = NEQ - 1;
   constexpr TabLayout L = tab_layout<NEQ>();
   const int lane = lane_id();
   const bool sp = lane < nsp;
-  gchar* tab = fresh(m.tab);
   const double rho = wave_sum(sp ? y : 0.0);
   const double T = bcast(y, nsp) / rho;
   double h, cp;
-  thermo_lane<NEQ>(tab, m.tmid, T, lane, h, cp);
+  thermo_lane<NEQ>(tab, tmid, T, lane, h, cp);
   const double Ck = sp ? y * ld<double>(tab, L.iw + lane * 8) : 0.0;
   const double cpm = wave_sum(sp ? (y / rho) * cp : 0.0);
   const double ctot = wave_sum(Ck);
-  const double cdot = prodrates_wave<NEQ>(m, S, Ck, ctot, T, p_gp);
+  const double cdot = prodrates_wave<NEQ>(tab, S, Ck, ctot, T, p_gp);
   const double rk = sp ? cdot * ld<double>(tab, L.w + lane * 8) : 0.0;
   const double tmp = wave_sum(-h * rk);
   return sp ? rk : (lane == nsp ? tmp / cpm : 0.0);
@@ -328,17 +332,11 @@ __device__ __forceinline__ double rhs_wave(const DevMech& m, RhsLds<NEQ>& S, dou
 
 // Out-of-line right-hand side for the integrator: one copy of the code
 // serves every call site (the solver calls it from six places), which keeps
-// the integrator's instruction footprint within the instruction cache.  Its
-// LDS scratch is allocated here, once per workgroup.
+// the integrator's instruction footprint within the instruction cache.
 template <int NEQ>
-__device__ __attribute__((noinline)) double rhs_call(const char* tab, double tmid, double y, double p_gp) {
-  __shared__ RhsLds<NEQ> S;
-  // arguments arrive in vector registers; the table base is wave-uniform
-  const uint64_t a = (uint64_t)(uintptr_t)tab;
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a >> 32));
-  const DevMech m{(const char*)(uintptr_t)(((uint64_t)hi << 32) | lo), NEQ - 1, tmid};
-  return rhs_wave<NEQ>(m, S, y, p_gp);
+__device__ __attribute__((noinline)) double rhs_call(tchar* tab, TGR_LDS RhsLds<NEQ>* S, double tmid, double y,
+                                                     double p_gp) {
+  return rhs_wave<NEQ>(tab, *S, tmid, y, p_gp);
 }
 
 }  // namespace tgr

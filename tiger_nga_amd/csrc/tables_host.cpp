@@ -18,10 +18,11 @@ static int seterr(std::string* err, const char* msg) {
 // Builds the table block (layout tgr::TabLayout for the mechanism's
 // compile-time shape): reactions re-ordered fall-off first (so only the first
 // 64-reaction round of a wave diverges), species padded to 64 lanes, the
-// third-body correction lists and the CDOT gather schedule.
+// third-body correction lists and the CDOT gather schedule, every list padded
+// to its row length with entries that leave the result unchanged.
 int build_table_blob(const tgr_mech_tables* t, std::vector<char>& blob, tgr_mech_info* info, std::string* err) {
   const int nsp = t->nsp, nr = t->nrxn;
-  tgr::TabLayout L(0, 0, 0, 0, 0, 0);
+  TabLayout L(0, 0, 0, 0, 0, 0);
   switch (nsp + 1) {
     case 10: L = tgr::tab_layout<10>(); break;
     case 37: L = tgr::tab_layout<37>(); break;
@@ -58,56 +59,71 @@ int build_table_blob(const tgr_mech_tables* t, std::vector<char>& blob, tgr_mech
   }
   if (nr != L.nrxn || nfo != L.nfo || t->n_third != L.nthird || S != L.gmax || maxe != L.maxe)
     return seterr(err, "mechanism tables do not match the compiled kernel shape");
+  if (nsp > kZero || t->n_third > kZero) return seterr(err, "too many species or third-body sums for 64 lanes");
+  if (nr >= L.rounds * 64) return seterr(err, "no padding slot in the last reaction round");
+  const int qzero = L.rounds * 64 - 1;  // rate-of-progress slot of a padding reaction (always 0)
 
   blob.assign(L.bytes, 0);
   auto D = [&](size_t off) -> double& { return *reinterpret_cast<double*>(&blob[off]); };
   auto I = [&](size_t off) -> int& { return *reinterpret_cast<int*>(&blob[off]); };
-  for (int g = 0; g < nr; ++g) {
-    const tgr_rxn& x = t->rxn[order[g]];
-    const size_t rx = L.rx + (size_t)g * 64;
-    for (int q = 0; q < 3; ++q) {
-      D(rx + 8 * q) = x.f[q];
-      D(rx + 24 + 8 * q) = x.b[q];
-    }
-    if (x.m + 1 > 127 || x.m + 1 < 0) return seterr(err, "third-body index out of range");
-    const int fo = (x.kind == 2) ? g : 0;
-    I(rx + 48) = (x.kind & 3) | ((x.rev & 1) << 2) | ((x.nr & 3) << 3) | ((x.np & 3) << 5) |
-                 (((x.m + 1) & 127) << 7) | ((fo & 255) << 14);
-    int rs = 0, ps = 0;
-    for (int q = 0; q < 3; ++q) {
-      rs |= (x.reac[q] < 0 ? 0 : x.reac[q]) << (8 * q);
-      ps |= (x.prod[q] < 0 ? 0 : x.prod[q]) << (8 * q);
-    }
-    I(rx + 52) = rs;
-    I(rx + 56) = ps;
-    if (x.kind == 2) {
-      const size_t f = L.fo + (size_t)g * 128;
-      int kinds = 0;
+  // reactions; padding lanes: A = 0 and every factor on the 1.0 slot
+  for (int g = 0; g < L.rounds * 64; ++g) {
+    const size_t rx = L.rx + (size_t)(g / 64) * kRxRound, dl = (size_t)(g % 64) * 8, il = (size_t)(g % 64) * 4;
+    int i0 = kOne | kOne << 8 | kOne << 16 | kOne << 24, i1 = kOne | kOne << 8 | kOne << 16;
+    if (g < nr) {
+      const tgr_rxn& x = t->rxn[order[g]];
+      if (x.nr > 3 || x.np > 3) return seterr(err, "more than three concentration factors");
       for (int q = 0; q < 3; ++q) {
-        D(f + 8 * q) = x.f0[q];
-        D(f + 24 + 8 * q) = x.b0[q];
-        D(f + 48 + 8 * q) = x.fc_c[q];
-        D(f + 72 + 8 * q) = x.fc_d[q];
-        kinds |= (x.fc_kind[q] & 1) << q;
+        D(rx + q * kRow + dl) = x.f[q];
+        D(rx + (3 + q) * kRow + dl) = x.rev ? x.b[q] : 0.0;
       }
-      I(f + 96) = x.nfc;
-      I(f + 100) = kinds;
+      auto put = [](int& w, int q, int v) { w = (w & ~(255 << (8 * q))) | (v << (8 * q)); };
+      for (int q = 0; q < x.nr; ++q) put(i0, q, x.reac[q]);
+      for (int q = 0; q < x.np; ++q) put(i1, q, x.prod[q]);
+      if (x.kind == 1) {
+        if (x.m < 0 || x.m >= t->n_third) return seterr(err, "third-body index out of range");
+        put(i0, 3, x.m);
+      }
+      if (x.kind == 2) {
+        if (g >= 64) return seterr(err, "more than 64 fall-off reactions");
+        i1 |= 1 << 24;
+        const size_t f = L.fo + dl;
+        int kinds = 0;
+        for (int q = 0; q < 3; ++q) {
+          D(f + q * kRow) = x.f0[q];
+          D(f + (3 + q) * kRow) = x.rev ? x.b0[q] : 0.0;
+          if (q < x.nfc) {
+            D(f + (6 + q) * kRow) = x.fc_c[q];
+            // exp(-T / d) as exp(-(1/d) T); exp(-d / T) as exp(-d (1/T))
+            D(f + (9 + q) * kRow) = x.fc_kind[q] ? x.fc_d[q] : 1.0 / x.fc_d[q];
+            kinds |= (x.fc_kind[q] & 1) << q;
+          }
+        }
+        if (x.m >= t->n_third) return seterr(err, "third-body index out of range");
+        I(L.fo + 12 * kRow + il) = kinds | (x.m >= 0 ? x.m : kZero) << 8;
+      }
+      if (x.rev) i1 |= 1 << 25;
     }
+    I(rx + 6 * kRow + il) = i0;
+    I(rx + 6 * kRow + kIRow + il) = i1;
   }
   for (int rg = 0; rg < 2; ++rg)
     for (int k = 0; k < nsp; ++k)
-      for (int q = 0; q < 13; ++q) D(L.th + ((size_t)(rg * 13 + q) * 64 + k) * 8) = t->thermo[(size_t)(rg * nsp + k) * 13 + q];
+      for (int q = 0; q < 13; ++q) D(L.th + (size_t)(rg * 13 + q) * kRow + k * 8) = t->thermo[(size_t)(rg * nsp + k) * 13 + q];
   for (int k = 0; k < nsp; ++k) {
     D(L.w + 8 * k) = t->W[k];
     D(L.iw + 8 * k) = 1.0 / t->W[k];
   }
-  for (int j = 0; j < t->n_third; ++j) {
-    I(L.en + 4 * j) = (int)corr[j].size();
-    for (size_t e = 0; e < corr[j].size(); ++e) {
-      I(L.ek + (e * 64 + j) * 4) = corr[j][e].first;
-      D(L.ev + (e * 64 + j) * 8) = corr[j][e].second;
+  // third-body corrections; padding: weight 0 on the 1.0 slot
+  for (int j = 0; j < 64; ++j)
+    for (int e = 0; e < maxe; ++e) {
+      const bool on = j < t->n_third && e < (int)corr[j].size();
+      I(L.ek + e * kIRow + j * 4) = on ? corr[j][e].first : kOne;
+      D(L.ev + e * kRow + j * 8) = on ? corr[j][e].second : 0.0;
     }
-  }
+  // CDOT gather; padding: weight 0 on a padding reaction's slot
+  for (int ln = 0; ln < 64; ++ln)
+    for (int e = 0; e < S; ++e) I(L.gr + e * kIRow + ln * 4) = qzero;
   int lane = 0;
   for (int k = 0; k < nsp; ++k) {
     const int nk = (int)bysp[k].size();
@@ -116,10 +132,9 @@ int build_table_blob(const tgr_mech_tables* t, std::vector<char>& blob, tgr_mech
     I(L.sn + 4 * k) = chunks;
     for (int ch = 0; ch < chunks; ++ch, ++lane) {
       const int lo = ch * S, hi = std::min(nk, lo + S);
-      I(L.gc + 4 * lane) = hi - lo;
       for (int e = lo; e < hi; ++e) {
-        I(L.gr + ((size_t)(e - lo) * 64 + lane) * 4) = bysp[k][e].first;
-        D(L.gn + ((size_t)(e - lo) * 64 + lane) * 8) = bysp[k][e].second;
+        I(L.gr + (size_t)(e - lo) * kIRow + lane * 4) = bysp[k][e].first;
+        D(L.gn + (size_t)(e - lo) * kRow + lane * 8) = bysp[k][e].second;
       }
     }
   }

@@ -6,10 +6,13 @@
 #include "wave_emu.h"
 #define TGR_OPAQUE_PTR "+r"
 #define TGR_GLOBAL
+#define TGR_LDS
 #else
 #include <hip/hip_runtime.h>
 #define TGR_OPAQUE_PTR "+s"
 // pointee qualifier of device (HBM) memory: loads through it are global_load,
 // not flat_load
 #define TGR_GLOBAL __attribute__((address_space(1)))
+// pointee qualifier of LDS: ds_read / ds_write instead of flat accesses
+#define TGR_LDS __attribute__((address_space(3)))
 #endif
