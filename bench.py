@@ -66,6 +66,19 @@ def cpu_baseline(workload, ncell, nthreads):
             "single_core_value": n1 / el1}
 
 
+def traffic_record(workload):
+    """HBM bytes per launch of the step kernel for this workload, from the
+    committed rocprofv3 --pmc summary (FETCH_SIZE / WRITE_SIZE runs, scaled
+    by a copy kernel of known size); None when no summary covers it."""
+    import glob
+    for f in sorted(glob.glob(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "*_traffic.json"))):
+        with open(f) as fh:
+            rec = json.load(fh)
+        if rec.get("workload") == workload:
+            return rec
+    return None
+
+
 def main():
     a = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -147,6 +160,10 @@ def main():
         "solver": {k: stats[k] for k in ("nst", "nfe", "nfe_dq", "nje", "nsetups", "nni", "netf", "ncfn", "nlu",
                                          "failed")},
     }
+    tr = traffic_record(a.workload)
+    if tr:
+        out["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
+        out["roofline"]["traffic_source"] = tr["source"]
     if not a.no_cpu:
         nt = a.cpu_threads or min(16, len(os.sched_getaffinity(0)))
         out["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_cells, nt)

@@ -69,8 +69,8 @@ struct Bdf {
   double ewt, acor, y, tempv, ftemp;
   double* jw;  // saved Jacobian: jw[c * 64 + lane] = J(lane, c)
   double M[NEQ];
-  int pstep;
-  double rdiag;
+  int pstep, pivl;
+  double rdk;
   // wave-uniform control state
   double tn, h, hprime, hscale, eta, etamax, rl1, gamma, gammap, gamrat, crate, acnrm, saved_tq5;
   // small wave-uniform arrays indexed by the run-time order (LDS, every lane
@@ -103,6 +103,9 @@ struct Bdf {
   __device__ __forceinline__ void set_ewt(double yv) { ewt = act ? 1.0 / (rtol * fabs(yv) + atol) : 0.0; }
 
   // ---- dense linear algebra, one matrix row per lane --------------------
+  // After getrf: pstep = elimination step at which this lane's row was the
+  // pivot row (-1 on lanes >= NEQ); lane k of pivl / rdk holds the pivot lane
+  // of step k and 1 / pivot of step k.
   __device__ __forceinline__ int getrf() {
     pstep = act ? NEQ : -1;
 #pragma unroll
@@ -115,9 +118,10 @@ struct Bdf {
       const int p = __builtin_amdgcn_readfirstlane(bal ? (int)__builtin_ctzll(bal) : 0);
       const double piv = bcast(M[k], p);
       const double rp = 1.0 / piv;
-      if (lane == p) {
-        pstep = k;
-        rdiag = rp;
+      if (lane == p) pstep = k;
+      if (lane == k) {
+        pivl = p;
+        rdk = rp;
       }
       const bool elim = cand && lane != p;
       // multiplier, 0 on rows that are not eliminated (then the update
@@ -135,16 +139,14 @@ struct Bdf {
     double xo = 0.0;
 #pragma unroll
     for (int k = 0; k < NEQ; ++k) {
-      const uint64_t bal = __ballot(pstep == k);
-      const int p = __builtin_amdgcn_readfirstlane(bal ? (int)__builtin_ctzll(bal) : 0);
+      const int p = __builtin_amdgcn_readlane(pivl, k);
       const double yk = bcast(b, p);
       if (pstep > k) b -= M[k] * yk;
     }
 #pragma unroll
     for (int k = NEQ - 1; k >= 0; --k) {
-      const uint64_t bal = __ballot(pstep == k);
-      const int p = __builtin_amdgcn_readfirstlane(bal ? (int)__builtin_ctzll(bal) : 0);
-      const double xk = bcast(b, p) * bcast(rdiag, p);
+      const int p = __builtin_amdgcn_readlane(pivl, k);
+      const double xk = bcast(b, p) * bcast(rdk, k);
       if (pstep >= 0 && pstep < k) b -= M[k] * xk;
       xo = (lane == k) ? xk : xo;
     }
@@ -606,7 +608,8 @@ struct Bdf {
 #pragma unroll
     for (int c = 0; c < NEQ; ++c) M[c] = 0.0;
     pstep = -1;
-    rdiag = 0.0;
+    pivl = 0;
+    rdk = 0.0;
     acor = y = tempv = 0.0;
     zn[0][lane] = act ? sol : 0.0;
     set_ewt(zn[0][lane]);

@@ -7,7 +7,8 @@
 //   |k| <= 1100; e^r on |r| <= ln2/2 by its degree-13 Taylor polynomial in
 //   Horner form (truncation < 5e-18 relative); result = 2^k * p by ldexp.
 //   Inputs below -746 give 0 (through ldexp underflow), above 710 give +inf,
-//   NaN propagates.  Ops: 1 mul, 1 rint, 15 fma, 1 ldexp, 1 cvt.
+//   NaN propagates.  Ops: 1 mul, 1 rint, 15 fma, 1 ldexp, 1 cvt (plus the
+//   clamp: min, max and a NaN select).
 // log: x = 2^e m with m in [sqrt(1/2), sqrt(2)) (frexp); with f = m - 1 and
 //   s = f/(2 + f), ln m = 2 atanh(s) = 2s(1 + z/3 + z^2/5 + ...), z = s^2,
 //   |s| <= 0.1716, truncated after z^10 (< 1e-18 relative); ln x =
@@ -27,7 +28,8 @@ __host__ __device__ __forceinline__ double fexp(double x) {
   const double kLog2e = 1.4426950408889634074;
   const double kLn2Hi = 6.93147180369123816490e-01;  // 32 significant bits
   const double kLn2Lo = 1.90821492927058770002e-10;
-  double xc = x < -746.0 ? -746.0 : (x > 710.0 ? 710.0 : x);
+  // clamp (maxnum/minnum: a NaN becomes -746 here and is restored at the end)
+  const double xc = fmin(fmax(x, -746.0), 710.0);
   const double k = rint(xc * kLog2e);
   double r = fma(-k, kLn2Hi, xc);
   r = fma(-k, kLn2Lo, r);
@@ -45,7 +47,8 @@ __host__ __device__ __forceinline__ double fexp(double x) {
   p = fma(p, r, 0.5);
   p = fma(p, r, 1.0);
   p = fma(p, r, 1.0);
-  return ldexp(p, (int)k);
+  const double v = ldexp(p, (int)k);
+  return x == x ? v : x;
 }
 
 __host__ __device__ __forceinline__ double flog(double x) {
