@@ -69,7 +69,7 @@ struct Bdf {
   double ewt, acor, y, tempv, ftemp;
   double* jw;  // saved Jacobian: jw[c * 64 + lane] = J(lane, c)
   double M[NEQ];
-  int pstep, pivl;
+  int pivl;
   double rdk;
   // wave-uniform control state
   double tn, h, hprime, hscale, eta, etamax, rl1, gamma, gammap, gamrat, crate, acnrm, saved_tq5;
@@ -103,11 +103,14 @@ struct Bdf {
   __device__ __forceinline__ void set_ewt(double yv) { ewt = act ? 1.0 / (rtol * fabs(yv) + atol) : 0.0; }
 
   // ---- dense linear algebra, one matrix row per lane --------------------
-  // After getrf: pstep = elimination step at which this lane's row was the
-  // pivot row (-1 on lanes >= NEQ); lane k of pivl / rdk holds the pivot lane
-  // of step k and 1 / pivot of step k.
+  // getrf factors M with partial pivoting (rows stay in their lanes during
+  // the elimination; the pivot row is broadcast through LDS, the right-hand-
+  // side scratch being idle), then moves row p_k of the factors to lane k,
+  // so that the triangular solves run in pivot order with compile-time lane
+  // masks.  rdk: lane k holds 1 / pivot of step k; pivl: lane k holds p_k.
   __device__ __forceinline__ int getrf() {
-    pstep = act ? NEQ : -1;
+    TGR_LDS double* prow = S->q;
+    int pstep = act ? NEQ : -1;
 #pragma unroll
     for (int k = 0; k < NEQ; ++k) {
       const bool cand = (pstep == NEQ);
@@ -116,9 +119,13 @@ struct Bdf {
       if (!(amax > 0.0)) return k + 1;
       const uint64_t bal = __ballot(cand && a == amax);
       const int p = __builtin_amdgcn_readfirstlane(bal ? (int)__builtin_ctzll(bal) : 0);
-      const double piv = bcast(M[k], p);
-      const double rp = 1.0 / piv;
-      if (lane == p) pstep = k;
+      if (lane == p) {
+        pstep = k;
+#pragma unroll
+        for (int c = k; c < NEQ; ++c) prow[c] = M[c];
+      }
+      wave_sync();
+      const double rp = 1.0 / prow[k];
       if (lane == k) {
         pivl = p;
         rdk = rp;
@@ -129,25 +136,28 @@ struct Bdf {
       const double lf = elim ? M[k] * rp : 0.0;
       if (elim) M[k] = lf;
 #pragma unroll
-      for (int c = k + 1; c < NEQ; ++c) M[c] = fma(-bcast(M[c], p), lf, M[c]);
+      for (int c = k + 1; c < NEQ; ++c) M[c] = fma(-prow[c], lf, M[c]);
+      wave_sync();
     }
+#pragma unroll
+    for (int c = 0; c < NEQ; ++c) M[c] = __shfl(M[c], pivl);
     return 0;
   }
 
   // solve M x = b with the factors from getrf; b, x: one component per lane
+  // (b in equation order, x in unknown order)
   __device__ __forceinline__ double getrs(double b) {
+    b = __shfl(b, pivl);  // pivot order
+#pragma unroll
+    for (int k = 0; k < NEQ - 1; ++k) {
+      const double yk = bcast(b, k);
+      if (lane > k && lane < NEQ) b = fma(-M[k], yk, b);
+    }
     double xo = 0.0;
 #pragma unroll
-    for (int k = 0; k < NEQ; ++k) {
-      const int p = __builtin_amdgcn_readlane(pivl, k);
-      const double yk = bcast(b, p);
-      if (pstep > k) b -= M[k] * yk;
-    }
-#pragma unroll
     for (int k = NEQ - 1; k >= 0; --k) {
-      const int p = __builtin_amdgcn_readlane(pivl, k);
-      const double xk = bcast(b, p) * bcast(rdk, k);
-      if (pstep >= 0 && pstep < k) b -= M[k] * xk;
+      const double xk = bcast(b, k) * bcast(rdk, k);
+      if (lane < k) b = fma(-M[k], xk, b);
       xo = (lane == k) ? xk : xo;
     }
     return xo;
@@ -607,7 +617,6 @@ struct Bdf {
     for (int j = 0; j <= kQmaxAbs; ++j) zn[j][lane] = 0.0;
 #pragma unroll
     for (int c = 0; c < NEQ; ++c) M[c] = 0.0;
-    pstep = -1;
     pivl = 0;
     rdk = 0.0;
     acor = y = tempv = 0.0;
