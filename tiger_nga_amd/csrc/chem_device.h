@@ -146,27 +146,33 @@ __device__ __forceinline__ double bcast(double v, int src) {
 }
 
 // Sum over the 64 lanes with the row_shr / row_bcast DPP tree (total lands
-// in lane 63), returned wave-uniform.  Fixed order: deterministic.
+// in lane 63), returned wave-uniform.  Every level writes all lanes, lanes
+// without a source adding 0 (bound_ctrl): lanes other than 63 then hold
+// partial sums that differ from a masked tree, but lane 63 sees the same
+// operands in the same order, so the result is the masked tree's, bit for
+// bit, without the register copies that masked levels need.  Fixed order:
+// deterministic.
 __device__ __forceinline__ double wave_sum(double v) {
   double s = v + dpp<0x111, 0xf, 0xf, true>(0.0, v);  // row_shr:1
   s = s + dpp<0x112, 0xf, 0xf, true>(0.0, v);         // row_shr:2
   s = s + dpp<0x113, 0xf, 0xf, true>(0.0, v);         // row_shr:3
-  s = s + dpp<0x114, 0xf, 0xe, false>(0.0, s);        // row_shr:4, banks 1-3
-  s = s + dpp<0x118, 0xf, 0xc, false>(0.0, s);        // row_shr:8, banks 2-3
-  s = s + dpp<0x142, 0xa, 0xf, false>(0.0, s);        // row_bcast:15, rows 1,3
-  s = s + dpp<0x143, 0xc, 0xf, false>(0.0, s);        // row_bcast:31, rows 2,3
+  s = s + dpp<0x114, 0xf, 0xf, true>(0.0, s);         // row_shr:4
+  s = s + dpp<0x118, 0xf, 0xf, true>(0.0, s);         // row_shr:8
+  s = s + dpp<0x142, 0xf, 0xf, true>(0.0, s);         // row_bcast:15
+  s = s + dpp<0x143, 0xf, 0xf, true>(0.0, s);         // row_bcast:31
   return bcast(s, 63);
 }
 
-// Maximum over the 64 lanes (same tree; lanes without a source keep their value).
+// Maximum over the 64 lanes of values >= -1 (same tree; lanes without a
+// source take 0, which never reaches lane 63).
 __device__ __forceinline__ double wave_max(double v) {
-  double s = fmax(v, dpp<0x111, 0xf, 0xf, false>(v, v));
-  s = fmax(s, dpp<0x112, 0xf, 0xf, false>(v, v));
-  s = fmax(s, dpp<0x113, 0xf, 0xf, false>(v, v));
-  s = fmax(s, dpp<0x114, 0xf, 0xe, false>(s, s));
-  s = fmax(s, dpp<0x118, 0xf, 0xc, false>(s, s));
-  s = fmax(s, dpp<0x142, 0xa, 0xf, false>(s, s));
-  s = fmax(s, dpp<0x143, 0xc, 0xf, false>(s, s));
+  double s = fmax(v, dpp<0x111, 0xf, 0xf, true>(0.0, v));
+  s = fmax(s, dpp<0x112, 0xf, 0xf, true>(0.0, v));
+  s = fmax(s, dpp<0x113, 0xf, 0xf, true>(0.0, v));
+  s = fmax(s, dpp<0x114, 0xf, 0xf, true>(0.0, s));
+  s = fmax(s, dpp<0x118, 0xf, 0xf, true>(0.0, s));
+  s = fmax(s, dpp<0x142, 0xf, 0xf, true>(0.0, s));
+  s = fmax(s, dpp<0x143, 0xf, 0xf, true>(0.0, s));
   return bcast(s, 63);
 }
 
