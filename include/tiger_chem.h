@@ -105,6 +105,17 @@ int tgr_chem_prodrates_device(tgr_chem *ctx, int64_t n, const double *C, const d
  * sol[neq][n] = [rho*Y, rho*T] -> rhs[neq][n] at pressure p_gp. */
 int tgr_chem_rhs_device(tgr_chem *ctx, int64_t n, double p_gp, const double *sol, double *rhs, void *stream);
 
+/* Temperature from mixture enthalpy for n states, device arrays, SoA:
+ * Y[nsp][n] mass fractions, hmix[n] J/kg, T0[n] K starting guesses (NULL or
+ * <= 0: 1500 K) -> T[n] K, iters[n] (may be NULL): Newton iterations, -1 when
+ * not converged in 50 or for a non-finite h.  No counterpart in the reference:
+ * the FP64 inversion of COMPTHERMODATA's h(T) (HYDROGENF.f90:514-673) used to
+ * check a predicted state's enthalpy; Newton on sum_k Y_k h_k(T) = h with cp as
+ * derivative, iterates clamped to the transport range [50, 5000] K
+ * (finitechem.f90:7-8), converged at |dT| <= 1e-10 T. */
+int tgr_chem_temperature_device(tgr_chem *ctx, int64_t n, const double *Y, const double *hmix, const double *T0,
+                                double *T, int *iters, void *stream);
+
 /* Tuning knobs (no counterpart in the reference):
  *   "grid"        size of the persistent grid in workgroups of 8 waves (0: occupancy x CUs);
  *                 every wave pulls cells

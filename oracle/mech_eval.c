@@ -182,3 +182,53 @@ int orc_rhs(int mech, double p_gp, const double *sol, double *rhs) {
   orc_rhs_t(t, p_gp, sol, rhs);
   return 0;
 }
+
+/* Temperature from mixture enthalpy (no routine in the reference; the FP64
+ * inversion of COMPTHERMODATA's h(T) that a surrogate's enthalpy check needs).
+ * Newton on hmix(T) = sum_k Y_k h_k(T) = h with cpmix(T) = sum_k Y_k cp_k(T)
+ * as the derivative, the thermo range chosen by T as in COMPTHERMODATA, each
+ * iterate clamped to [ORC_T_LOW, ORC_T_HIGH] (finitechem.f90:7-8 Tmin/Tmax),
+ * stop when |T_new - T| <= 1e-10 T_new, at most ORC_T_MAXIT iterations; three
+ * consecutive crossings of the range switch (h inside the jump of h(T) there)
+ * end at T = tmid.
+ * T0 <= 0 or not finite starts from 1500 K.  Returns the iteration count on
+ * convergence, -1 otherwise (T then holds the last iterate). */
+int orc_temperature_t(const tgr_mech_tables *t, const double *Y, double hmix, double T0, double *T) {
+  double h[64], cp[64];
+  if (!isfinite(hmix)) {
+    *T = NAN;
+    return -1;
+  }
+  double tc = (T0 > 0.0 && isfinite(T0)) ? T0 : 1500.0;
+  int flips = 0;
+  for (int it = 1; it <= ORC_T_MAXIT; ++it) {
+    orc_thermo_t(t, tc, h, cp);
+    double hm = 0.0, cpm = 0.0;
+    for (int k = 0; k < t->nsp; ++k) {
+      hm += Y[k] * h[k];
+      cpm += Y[k] * cp[k];
+    }
+    double tn = fmin(fmax(tc - (hm - hmix) / cpm, ORC_T_LOW), ORC_T_HIGH);
+    int done = fabs(tn - tc) <= 1.0e-10 * tn;
+    /* h(T) jumps at the range switch: an h inside the jump has no root and
+     * Newton alternates across the switch; the answer is then the switch */
+    flips = ((tn > t->tmid) != (tc > t->tmid)) ? flips + 1 : 0;
+    if (flips >= 3) {
+      tn = t->tmid;
+      done = 1;
+    }
+    tc = tn;
+    if (done) {
+      *T = tc;
+      return it;
+    }
+  }
+  *T = tc;
+  return -1;
+}
+
+int orc_temperature(int mech, const double *Y, double hmix, double T0, double *T) {
+  const tgr_mech_tables *t = orc_tables(mech);
+  if (!t) return -2;
+  return orc_temperature_t(t, Y, hmix, T0, T);
+}

@@ -61,6 +61,12 @@ int orc_prodrates(int mech, const double *C, double T, double P, double *cdot,
                   double *K, double *W, double *M);
 /* COMPTHERMODATA: h[nsp] J/kg, cp[nsp] J/kg/K */
 int orc_thermo(int mech, double T, double *h, double *cp);
+/* temperature from mixture enthalpy (Newton; no reference routine): returns
+ * the iteration count, -1 if not converged, -2 on a bad mechanism id */
+#define ORC_T_LOW 50.0
+#define ORC_T_HIGH 5000.0
+#define ORC_T_MAXIT 50
+int orc_temperature(int mech, const double *Y, double hmix, double T0, double *T);
 /* finitechem_compute_rhs: sol = [rho*Y (nsp), rho*T], p_gp = 0.5(P_th+P_th_old) */
 int orc_rhs(int mech, double p_gp, const double *sol, double *rhs);
 /* one cell, t: 0 -> dt, sol in/out (neq = nsp+1) */

@@ -58,6 +58,7 @@ def lib():
                                            ctypes.c_double, ctypes.POINTER(Grid), _dp, _dp, ctypes.c_void_p,
                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Stats), ctypes.c_int]
         L.orc_molar_mass.argtypes = [ctypes.c_int, _dp]
+        L.orc_temperature.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, _dp]
         _lib = L
     return _lib
 
@@ -81,6 +82,14 @@ def thermo(mech, T, nsp):
     h, cp = np.zeros(nsp), np.zeros(nsp)
     lib().orc_thermo(MECH_ID[mech], T, _p(h), _p(cp))
     return h, cp
+
+
+def temperature(mech, Y, hmix, T0=0.0):
+    """(T, iterations) of the enthalpy-to-temperature Newton inversion."""
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    T = np.zeros(1)
+    it = lib().orc_temperature(MECH_ID[mech], _p(Y), float(hmix), float(T0), _p(T))
+    return T[0], it
 
 
 def rhs(mech, sol, p_gp):

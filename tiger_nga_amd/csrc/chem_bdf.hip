@@ -112,7 +112,72 @@ __global__ __launch_bounds__(64) void rhs_lanes_kernel(int64_t n, double p_gp, c
   for (int k = 0; k < NEQ; ++k) f[k * n + c] = o[k];
 }
 
-__global__ void reset_counter_kernel(unsigned int* q) { *q = 0u; }
+// Temperature from mixture enthalpy, lane per state (no routine in the
+// reference: the FP64 inversion of COMPTHERMODATA's h(T), HYDROGENF.f90:514-673,
+// that a surrogate's enthalpy check needs).  Newton on
+// sum_k Y_k h_k(T) = h with sum_k Y_k cp_k(T) as the derivative, the range
+// chosen by T > tmid, iterates clamped to [50, 5000] K (finitechem.f90:7-8),
+// stop at |dT| <= 1e-10 T, at most 50 iterations; T0 <= 0 starts at 1500 K;
+// three consecutive crossings of the range switch end at T = tmid.
+// iters: iterations on convergence, -1 otherwise.
+constexpr double kTLow = 50.0, kTHigh = 5000.0, kTGuess = 1500.0;
+constexpr int kTMaxIt = 50;
+
+template <int NEQ>
+__global__ __launch_bounds__(64) void temperature_kernel(DevMech m, int64_t n, const double* Y, const double* hmix,
+                                                         const double* T0, double* T, int* iters) {
+  constexpr int nsp = NEQ - 1;
+  constexpr TabLayout L = tab_layout<NEQ>();
+  const int64_t c = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (c >= n) return;
+  double y[nsp];
+#pragma unroll
+  for (int k = 0; k < nsp; ++k) y[k] = Y[k * n + c];
+  const double ht = hmix[c];
+  double t = T0 ? T0[c] : 0.0;
+  if (!(t > 0.0) || !isfinite(t)) t = kTGuess;
+  int res = -1, flips = 0;
+  if (isfinite(ht)) {
+    for (int it = 1; it <= kTMaxIt; ++it) {
+      const TGR_GLOBAL double* th = (const TGR_GLOBAL double*)(m.tab + L.th + (t > m.tmid ? 0 : 13 * kRow));
+      double hs = 0.0, cs = 0.0;
+#pragma unroll
+      for (int k = 0; k < nsp; ++k) {
+        const TGR_GLOBAL double* cf = th + k;  // coefficient q of species k at cf[64 q]
+        const double hk =
+            cf[0] * (t * (cf[64] + t * (cf[128] + t * (cf[192] + t * (cf[256] + t * cf[320])))) + cf[384]);
+        const double ck = cf[448] * (cf[512] + t * (cf[576] + t * (cf[640] + t * (cf[704] + t * cf[768]))));
+        hs += y[k] * hk;
+        cs += y[k] * ck;
+      }
+      double tn = fmin(fmax(t - (hs - ht) / cs, kTLow), kTHigh);
+      bool done = fabs(tn - t) <= 1.0e-10 * tn;
+      // h(T) jumps at the range switch: an h inside the jump has no root and
+      // Newton alternates across the switch; the answer is then the switch
+      flips = ((tn > m.tmid) != (t > m.tmid)) ? flips + 1 : 0;
+      if (flips >= 3) {
+        tn = m.tmid;
+        done = true;
+      }
+      t = tn;
+      if (done) {
+        res = it;
+        break;
+      }
+    }
+  } else {
+    t = ht;
+  }
+  T[c] = t;
+  if (iters) iters[c] = res;
+}
+
+// Clears the work counter and the per-call totals: a kernel rather than a
+// memset, so that a caller may capture the whole step in a HIP graph.
+__global__ void reset_counter_kernel(unsigned int* q, unsigned long long* totals) {
+  if (threadIdx.x == 0) *q = 0u;
+  if (totals && threadIdx.x < TGR_NCOUNT) totals[threadIdx.x] = 0ull;
+}
 
 // grid of the component kernels: enough workgroups to fill the GPU, each
 // looping over states (the tables are staged once per workgroup)
@@ -141,7 +206,7 @@ extern "C" int tgr_step_occupancy(int neq) {
 
 extern "C" int tgr_launch_step(const StepParams* p, int neq, int grid, hipStream_t s) {
   if (!p->queue || !p->ws) return -1;
-  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, p->queue);
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, p->queue, p->totals);
   switch (neq) {
     case 10: hipLaunchKernelGGL(step_kernel<10>, dim3(grid), dim3(kBlock), 0, s, *p); break;
     case 37: hipLaunchKernelGGL(step_kernel<37>, dim3(grid), dim3(kBlock), 0, s, *p); break;
@@ -183,6 +248,19 @@ extern "C" int tgr_launch_rhs(const DevMech* m, int64_t n, double p_gp, const do
     case 10: hipLaunchKernelGGL(rhs_kernel<10>, g, b, 0, s, *m, n, p_gp, y, f); break;
     case 37: hipLaunchKernelGGL(rhs_kernel<37>, g, b, 0, s, *m, n, p_gp, y, f); break;
     case 40: hipLaunchKernelGGL(rhs_kernel<40>, g, b, 0, s, *m, n, p_gp, y, f); break;
+    default: return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+extern "C" int tgr_launch_temperature(const DevMech* m, int64_t n, const double* Y, const double* hmix,
+                                      const double* T0, double* T, int* iters, hipStream_t s) {
+  if (n <= 0) return 0;
+  const dim3 grid((unsigned)((n + 63) / 64));
+  switch (m->nsp + 1) {
+    case 10: hipLaunchKernelGGL(temperature_kernel<10>, grid, dim3(64), 0, s, *m, n, Y, hmix, T0, T, iters); break;
+    case 37: hipLaunchKernelGGL(temperature_kernel<37>, grid, dim3(64), 0, s, *m, n, Y, hmix, T0, T, iters); break;
+    case 40: hipLaunchKernelGGL(temperature_kernel<40>, grid, dim3(64), 0, s, *m, n, Y, hmix, T0, T, iters); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;

@@ -170,11 +170,12 @@ int run_step(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const
   if (rc0) return rc0;
   p.queue = c->d_queue;
   p.ws = (double*)c->b_ws.p;
-  HIP_TRY(hipMemsetAsync(c->d_totals, 0, sizeof(unsigned long long) * TGR_NCOUNT, s));
-  HIP_TRY(hipEventRecord(c->ev0, s));
+  // with stats == NULL the call enqueues two kernels and nothing else (no
+  // memset, no event), so that it can be captured in a HIP graph
+  if (stats) HIP_TRY(hipEventRecord(c->ev0, s));
   int rc = tgr_launch_step(&p, c->neq, grid, s);
   if (rc) return fail(TGR_E_HIP, "step kernel launch failed");
-  HIP_TRY(hipEventRecord(c->ev1, s));
+  if (stats) HIP_TRY(hipEventRecord(c->ev1, s));
   if (stats) {
     unsigned long long tot[TGR_NCOUNT];
     HIP_TRY(hipMemcpyAsync(tot, c->d_totals, sizeof(tot), hipMemcpyDeviceToHost, s));
@@ -358,6 +359,14 @@ int tgr_chem_prodrates_device(tgr_chem* c, int64_t n, const double* C, const dou
   if (!c || (n > 0 && (!C || !T || !P || !cdot))) return fail(TGR_E_ARG, "NULL argument");
   if (n < 0 || n > 0x7fffffff) return fail(TGR_E_ARG, "bad n");
   if (tgr_launch_prodrates(&c->dm, n, C, T, P, cdot, (hipStream_t)stream)) return fail(TGR_E_HIP, "launch");
+  return 0;
+}
+
+int tgr_chem_temperature_device(tgr_chem* c, int64_t n, const double* Y, const double* hmix, const double* T0,
+                                double* T, int* iters, void* stream) {
+  if (!c || (n > 0 && (!Y || !hmix || !T))) return fail(TGR_E_ARG, "NULL argument");
+  if (n < 0 || n > 0x7fffffff) return fail(TGR_E_ARG, "bad n");
+  if (tgr_launch_temperature(&c->dm, n, Y, hmix, T0, T, iters, (hipStream_t)stream)) return fail(TGR_E_HIP, "launch");
   return 0;
 }
 

@@ -13,5 +13,7 @@ int tgr_launch_step(const tgr::StepParams* p, int neq, int grid, hipStream_t s);
 int tgr_launch_prodrates(const tgr::DevMech* m, int64_t n, const double* C, const double* T, const double* P,
                          double* cdot, hipStream_t s);
 int tgr_launch_rhs(const tgr::DevMech* m, int64_t n, double p_gp, const double* y, double* f, hipStream_t s);
+int tgr_launch_temperature(const tgr::DevMech* m, int64_t n, const double* Y, const double* hmix, const double* T0,
+                           double* T, int* iters, hipStream_t s);
 int tgr_launch_rhs_lanes(int neq, int64_t n, double p_gp, const double* y, double* f, hipStream_t s);
 }

@@ -129,6 +129,7 @@ def load_library(path: str = LIB_PATH):
     lib.tgr_chem_source_chemistry_device.argtypes = step_args + [vp, ctypes.POINTER(_Stats)]
     lib.tgr_chem_prodrates_device.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp, vp]
     lib.tgr_chem_rhs_device.argtypes = [vp, ctypes.c_int64, ctypes.c_double, vp, vp, vp]
+    lib.tgr_chem_temperature_device.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp, vp, vp]
     lib.tgr_chem_set_option.argtypes = [vp, ctypes.c_char_p, ctypes.c_int64]
     lib.tgr_chem_last_error.restype = ctypes.c_char_p
     _lib = lib
@@ -258,6 +259,18 @@ class FiniteChem:
         _check(self.lib, self.lib.tgr_chem_prodrates_device(self._h, n, _ptr(C), _ptr(T), _ptr(P), _ptr(out),
                                                             stream), "prodrates")
         return out
+
+    def temperature(self, Y, hmix, T0=None, stream=None):
+        """(T[n], iters[n]) from mass fractions Y[nsp, n] and mixture enthalpy
+        hmix[n] (J/kg) -- device tensors; Newton inversion of the NASA-7 h(T)."""
+        import torch
+        n = hmix.shape[0]
+        assert Y.shape == (self.nsp, n) and Y.is_contiguous() and hmix.is_contiguous()
+        T = torch.empty_like(hmix)
+        it = torch.empty(n, dtype=torch.int32, device=hmix.device)
+        _check(self.lib, self.lib.tgr_chem_temperature_device(self._h, n, _ptr(Y), _ptr(hmix), _ptr(T0), _ptr(T),
+                                                               _ptr(it), stream), "temperature")
+        return T, it
 
     def compute_rhs(self, sol, p_gp: float, stream=None):
         """rhs[neq, n] of sol[neq, n] = [rho*Y, rho*T] -- device tensors."""
