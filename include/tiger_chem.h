@@ -105,6 +105,21 @@ int tgr_chem_prodrates_device(tgr_chem *ctx, int64_t n, const double *C, const d
  * sol[neq][n] = [rho*Y, rho*T] -> rhs[neq][n] at pressure p_gp. */
 int tgr_chem_rhs_device(tgr_chem *ctx, int64_t n, double p_gp, const double *sol, double *rhs, void *stream);
 
+/* Transport and equation-of-state properties of a block, DEVICE arrays in the
+ * layout of the step (RHO_ / SC_ Fortran order): for every cell of the array
+ * (ghosts included) with mask /= 1,
+ *   RHO_  = P_thermo W_mix / (R_cst T)                    finitechem_density (finitechem.f90:324-345)
+ *   VISC_ = Wilke mixture viscosity                        finitechem_viscosity (351-411)
+ *   DIFF_(..., species) = lambda / cp / Lewis_k, DIFF_(..., isc_T) = lambda / cp and, when
+ *   isc_zmix > 0, DIFF_(..., isc_zmix) = lambda / cp       finitechem_diffusivity (414-461)
+ * with GETMUCOEFF / GETKOVEREPS (HYDROGENF.f90:453-496) and finitechem_omegamu
+ * (466-486).  Lewis: device array [nsp] or NULL (all 1, finitechem.f90:295);
+ * any of RHO_, VISC_, DIFF_ may be NULL.  The border exchange that follows in
+ * the reference (boundary_update_border) stays with the caller. */
+int tgr_chem_properties_device(tgr_chem *ctx, double p_thermo, const tgr_grid *g, const double *SC_,
+                               const int *mask, const double *Lewis, int isc_zmix, double *RHO_, double *VISC_,
+                               double *DIFF_, void *stream);
+
 /* Temperature from mixture enthalpy for n states, device arrays, SoA:
  * Y[nsp][n] mass fractions, hmix[n] J/kg, T0[n] K starting guesses (NULL or
  * <= 0: 1500 K) -> T[n] K, iters[n] (may be NULL): Newton iterations, -1 when

@@ -232,3 +232,88 @@ int orc_temperature(int mech, const double *Y, double hmix, double T0, double *T
   if (!t) return -2;
   return orc_temperature_t(t, Y, hmix, T0, T);
 }
+
+/* finitechem_omegamu (combustion/finitechem.f90:466-486): FlameMaster fit of
+ * the collision integral */
+static double omegamu(double t) {
+  const double m1 = 3.3530622607, m2 = 2.53272006, m3 = 2.9024238575, m4 = 0.11186138893, m5 = 0.8662326188,
+               m6 = 1.3913958626, m7 = 3.158490576, m8 = 0.18973411754, m9 = 0.00018682962894;
+  const double num = m1 + t * (m2 + t * (m3 + t * m4));
+  const double den = m5 + t * (m6 + t * (m7 + t * (m8 + t * m9)));
+  return num / den;
+}
+
+/* Transport and equation-of-state properties of one cell, restating
+ *   finitechem_W          finitechem.f90:65-74   (W_mix of clipped Y)
+ *   finitechem_density    finitechem.f90:324-345 (RHO = P_thermo W_mix / (R T))
+ *   finitechem_viscosity  finitechem.f90:351-411 (Wilke mixing of the species
+ *                          viscosities eta_k = mucoeff_k sqrt(T) / omegamu)
+ *   finitechem_diffusivity finitechem.f90:414-461 (mixture conductivity,
+ *                          DIFF = lambda / cp / Lewis, T and ZMIX: lambda / cp)
+ * with T clipped to [Tmin, Tmax] = [50, 5000] K for the transport parts
+ * (finitechem.f90:7-8) and R_cst = 8314.34 (exact double, finitechem.f90:15).
+ * Lewis may be NULL (all 1). */
+int orc_properties_t(const tgr_mech_tables *t, const double *Y, double Tcell, double P_thermo, const double *Lewis,
+                     double *visc, double *diff_sp, double *diff_T, double *rho) {
+  const int n = t->nsp;
+  const double *W = t->W;
+  const double R_cst = 8314.34;
+  double st = 0.0, sw = 0.0;
+  for (int k = 0; k < n; ++k) {
+    const double yt = fmin(fmax(Y[k], 0.0), 1.0);
+    st += yt;
+    sw += yt / W[k];
+  }
+  const double Wmix = st / sw;
+  if (rho) *rho = P_thermo * Wmix / (R_cst * Tcell);
+  const double temp = fmin(fmax(Tcell, 50.0), 5000.0);
+  double eta[64];
+  for (int k = 0; k < n; ++k) eta[k] = t->mucoeff[k] * sqrt(temp) / omegamu(temp * t->koveps[k]);
+  if (visc) {
+    double sum_2 = 0.0;
+    for (int k = 0; k < n; ++k) {
+      double sum_1 = 0.0;
+      for (int j = 0; j < n; ++j) {
+        if (j == k) continue;
+        double phi = (1.0 / sqrt(8.0)) * pow(1.0 + (W[k] / W[j]), -0.5);
+        const double temp1 = pow(eta[k] / eta[j], 0.5) * pow(W[j] / W[k], 0.25);
+        phi = phi * ((1.0 + temp1) * (1.0 + temp1));
+        sum_1 = sum_1 + (Y[j] * Wmix * phi / W[j]);
+      }
+      sum_1 = sum_1 + Y[k] * Wmix / W[k];
+      if (Y[k] > 0.0) sum_2 = sum_2 + (Y[k] * Wmix * eta[k] / (W[k] * sum_1));
+    }
+    *visc = sum_2;
+  }
+  if (diff_sp || diff_T) {
+    double h[64], cp[64], cond[64];
+    orc_thermo_t(t, temp, h, cp);
+    for (int k = 0; k < n; ++k) cond[k] = eta[k] * (cp[k] + 1.25 * R_cst / W[k]);
+    double s1 = 0.0, s2 = 0.0, cpm = 0.0;
+    for (int k = 0; k < n; ++k) {
+      s1 = s1 + (Y[k] * Wmix / (cond[k] * W[k]));
+      s2 = s2 + (Y[k] * cond[k] * Wmix / W[k]);
+    }
+    const double lambda = 0.5 * (s2 + (1.0 / s1));
+    for (int k = 0; k < n; ++k) cpm += Y[k] * cp[k];
+    if (diff_sp)
+      for (int k = 0; k < n; ++k) diff_sp[k] = (lambda / cpm) / (Lewis ? Lewis[k] : 1.0);
+    if (diff_T) *diff_T = lambda / cpm;
+  }
+  return 0;
+}
+
+int orc_properties(int mech, const double *Y, double Tcell, double P_thermo, const double *Lewis, double *visc,
+                   double *diff_sp, double *diff_T, double *rho) {
+  const tgr_mech_tables *t = orc_tables(mech);
+  if (!t) return -1;
+  return orc_properties_t(t, Y, Tcell, P_thermo, Lewis, visc, diff_sp, diff_T, rho);
+}
+
+int orc_transport_fits(int mech, double *mucoeff, double *koveps) {
+  const tgr_mech_tables *t = orc_tables(mech);
+  if (!t) return -1;
+  memcpy(mucoeff, t->mucoeff, sizeof(double) * t->nsp);
+  memcpy(koveps, t->koveps, sizeof(double) * t->nsp);
+  return 0;
+}

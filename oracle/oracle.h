@@ -25,6 +25,12 @@
  *                   invariants, not against CVODE's own step sequence.
  *   orc_source_chemistry  finitechem.f90:962-1071 (pack, integrate, unpack,
  *                   heat release and H2O/CO/CO2 source diagnostics).
+ *   orc_properties  finitechem.f90:65-74, 324-486 (W_mix, density, Wilke
+ *                   viscosity, conductivity / diffusivities, omegamu).  Pinned
+ *                   by the golden GETMUCOEFF / GETKOVEREPS values and closed
+ *                   forms (pure species, ideal gas).
+ *   orc_temperature enthalpy -> temperature Newton inversion (no reference
+ *                   routine), pinned by round trips.
  */
 #pragma once
 #include <stdint.h>
@@ -67,6 +73,11 @@ int orc_thermo(int mech, double T, double *h, double *cp);
 #define ORC_T_HIGH 5000.0
 #define ORC_T_MAXIT 50
 int orc_temperature(int mech, const double *Y, double hmix, double T0, double *T);
+/* finitechem_density / _viscosity / _diffusivity of one cell (Lewis may be NULL) */
+int orc_properties(int mech, const double *Y, double Tcell, double P_thermo, const double *Lewis, double *visc,
+                   double *diff_sp, double *diff_T, double *rho);
+/* GETMUCOEFF / GETKOVEREPS tables */
+int orc_transport_fits(int mech, double *mucoeff, double *koveps);
 /* finitechem_compute_rhs: sol = [rho*Y (nsp), rho*T], p_gp = 0.5(P_th+P_th_old) */
 int orc_rhs(int mech, double p_gp, const double *sol, double *rhs);
 /* one cell, t: 0 -> dt, sol in/out (neq = nsp+1) */

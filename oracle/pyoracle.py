@@ -59,6 +59,9 @@ def lib():
                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Stats), ctypes.c_int]
         L.orc_molar_mass.argtypes = [ctypes.c_int, _dp]
         L.orc_temperature.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, _dp]
+        L.orc_properties.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_void_p, _dp, _dp,
+                                     _dp, _dp]
+        L.orc_transport_fits.argtypes = [ctypes.c_int, _dp, _dp]
         _lib = L
     return _lib
 
@@ -90,6 +93,23 @@ def temperature(mech, Y, hmix, T0=0.0):
     T = np.zeros(1)
     it = lib().orc_temperature(MECH_ID[mech], _p(Y), float(hmix), float(T0), _p(T))
     return T[0], it
+
+
+def properties(mech, Y, T, P_thermo, Lewis=None):
+    """(visc, diff_sp[nsp], diff_T, rho) of one cell."""
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    v, dT, r = np.zeros(1), np.zeros(1), np.zeros(1)
+    d = np.zeros(Y.shape[0])
+    lw = None if Lewis is None else np.ascontiguousarray(Lewis, dtype=np.float64)
+    lib().orc_properties(MECH_ID[mech], _p(Y), float(T), float(P_thermo), None if lw is None else lw.ctypes.data,
+                         _p(v), _p(d), _p(dT), _p(r))
+    return v[0], d, dT[0], r[0]
+
+
+def transport_fits(mech, nsp):
+    mu, ke = np.zeros(nsp), np.zeros(nsp)
+    lib().orc_transport_fits(MECH_ID[mech], _p(mu), _p(ke))
+    return mu, ke
 
 
 def rhs(mech, sol, p_gp):

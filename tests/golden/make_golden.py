@@ -9,6 +9,7 @@ outputs as fixtures (``<mech>_mech.npz``):
   -> per-slot rate coefficients K, rates W, third-body sums M, CDOT
 * COMPTHERMODATA(H, CP, T)                       (HYDROGENF.f90:514-673)
 * GETMOLARMASS / GETSPECIESNAMES / GETNSPECIES / GETNREACTIONS
+* GETMUCOEFF / GETKOVEREPS                       (HYDROGENF.f90:453-496)
 
 States cover T in [300, 3000] K across the 1000 K polynomial switch (and
 exactly at it), 1 and 10 atm, log-uniform concentrations, exact zeros (M = 0
@@ -67,6 +68,9 @@ def main():
         nm = 64
         W_mol = np.zeros(nsp)
         lib.getmolarmass_(_ptr(W_mol))
+        mucoeff, koveps = np.zeros(nsp), np.zeros(nsp)
+        lib.getmucoeff_(_ptr(mucoeff))
+        lib.getkovereps_(_ptr(koveps))
         names = ctypes.create_string_buffer(20 * nsp)
         lib.getspeciesnames_(names, ctypes.c_long(20))
         names = [names.raw[20 * k:20 * k + 20].decode().strip() for k in range(nsp)]
@@ -91,7 +95,7 @@ def main():
             H[s], CP[s] = h, cp
         out = os.path.join(HERE, "%s_mech.npz" % mech)
         np.savez_compressed(out, C=C, T=T, P=P, CDOT=CDOT, K=K, W=W, M=M, T_thermo=Tt, H=H, CP=CP,
-                            W_mol=W_mol, names=np.array(names), nslot=nslot)
+                            W_mol=W_mol, mucoeff=mucoeff, koveps=koveps, names=np.array(names), nslot=nslot)
         print("%s: %d species, %d slots -> %s (%d bytes)" % (mech, nsp, nslot, out, os.path.getsize(out)))
 
 

@@ -1,0 +1,115 @@
+"""Transport and equation-of-state properties (finitechem_density,
+finitechem_viscosity, finitechem_diffusivity, finitechem_omegamu:
+combustion/finitechem.f90:324-486).  The oracle is pinned by the reference's
+own GETMUCOEFF / GETKOVEREPS values (golden vectors from the flang-built
+mechanism files), the ideal-gas law, tabulated gas viscosities and mixing
+bounds; the GPU kernel is checked against the oracle cell by cell."""
+import os
+
+import numpy as np
+import pytest
+
+from tiger_nga_amd import inputs
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+NSP = {"h2": 9, "gri30": 36, "dme39": 39}
+
+
+@pytest.mark.parametrize("mech", ["h2", "gri30", "dme39"])
+def test_transport_fits_match_reference(mech, oracle):
+    d = np.load(os.path.join(GOLD, "%s_mech.npz" % mech))
+    mu, ke = oracle.transport_fits(mech, NSP[mech])
+    np.testing.assert_array_equal(mu, d["mucoeff"])
+    np.testing.assert_array_equal(ke, d["koveps"])
+
+
+def _pure(mech, name):
+    from tiger_nga_amd.mech import meta
+    Y = np.zeros(NSP[mech])
+    Y[meta.index(mech, name)] = 1.0
+    return Y
+
+
+def test_ideal_gas_density(oracle):
+    from tiger_nga_amd.mech import meta
+    W = meta.molar_mass("gri30")
+    for name in ("N2", "CH4", "H2O"):
+        for T in (300.0, 1500.0):
+            _, _, _, rho = oracle.properties("gri30", _pure("gri30", name), T, 101325.0)
+            assert rho == pytest.approx(101325.0 * W[meta.index("gri30", name)] / (8314.34 * T), rel=1e-14)
+
+
+@pytest.mark.parametrize("name,T,mu_ref", [("N2", 300.0, 1.79e-5), ("N2", 1000.0, 4.15e-5), ("O2", 300.0, 2.07e-5),
+                                           ("CH4", 300.0, 1.12e-5), ("H2", 300.0, 8.96e-6)])
+def test_pure_gas_viscosity_is_tabulated_value(oracle, name, T, mu_ref):
+    """Wilke's rule reduces to the species viscosity for a pure gas; the
+    FlameMaster fits reproduce tabulated gas viscosities (CRC / NIST) to a
+    few per cent."""
+    v, _, _, _ = oracle.properties("gri30", _pure("gri30", name), T, 101325.0)
+    assert v == pytest.approx(mu_ref, rel=0.05)
+
+
+def test_mixture_viscosity_between_pure_values(oracle):
+    a, b = _pure("gri30", "N2"), _pure("gri30", "CH4")
+    for T in (400.0, 1200.0, 2000.0):
+        va = oracle.properties("gri30", a, T, 101325.0)[0]
+        vb = oracle.properties("gri30", b, T, 101325.0)[0]
+        for x in (0.1, 0.5, 0.9):
+            v = oracle.properties("gri30", x * a + (1 - x) * b, T, 101325.0)[0]
+            assert min(va, vb) <= v <= max(va, vb)
+
+
+def test_lewis_scaling_and_temperature_diffusivity(oracle):
+    mech, Y, T, rho, P, dt = inputs.states("gri30", n=3, seed=2)
+    Le = np.linspace(0.5, 2.0, NSP[mech])
+    _, d1, dT, _ = oracle.properties(mech, Y[0], T[0], P)
+    _, d2, dT2, _ = oracle.properties(mech, Y[0], T[0], P, Lewis=Le)
+    np.testing.assert_array_equal(d1, np.full(NSP[mech], dT))
+    np.testing.assert_allclose(d2 * Le, d1, rtol=1e-15)
+    assert dT2 == dT
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("wl", ["h2", "gri30", "dme39"])
+def test_properties_kernel_vs_oracle(wl, oracle):
+    import torch
+    from tiger_nga_amd.finitechem import FiniteChem, Grid
+    mech, Y, T, rho, P, dt = inputs.states(wl, n=6 * 5 * 4, seed=13)
+    RHO_, SC_, isc_sc, isc_T = inputs.fortran_fields(Y, T, rho, 6, 5, 4, nover=1, extra_scalars=1)
+    nscalar = SC_.shape[3]
+    g = Grid.block(6, 5, 4, nscalar, isc_sc, isc_T, nover=1)
+    mask = np.zeros(g.mask_shape, dtype=np.int32, order="F")
+    mask[2, 3] = 1
+    Le = np.linspace(0.7, 1.6, NSP[mech])
+    isc_z = nscalar  # the extra scalar plays ZMIX
+
+    def fe(a, dtype=torch.float64):
+        st = [1]
+        for s in a.shape[:-1]:
+            st.append(st[-1] * s)
+        t = torch.empty_strided(a.shape, tuple(st), dtype=dtype, device="cuda")
+        t.copy_(torch.from_numpy(a))
+        return t
+    S_d, M_d = fe(SC_), fe(mask, torch.int32)
+    R_d, V_d, D_d = fe(np.full(RHO_.shape, -7.0, order="F")), fe(np.full(RHO_.shape, -7.0, order="F")), \
+        fe(np.full(SC_.shape, -7.0, order="F"))
+    fc = FiniteChem(mech)
+    fc.properties(S_d, g, P, mask=M_d, Lewis=torch.tensor(Le, device="cuda"), isc_zmix=isc_z, RHO_=R_d, VISC_=V_d,
+                  DIFF_=D_d)
+    torch.cuda.synchronize()
+    R, V, D = R_d.cpu().numpy(), V_d.cpu().numpy(), D_d.cpu().numpy()
+    nsp, worst = NSP[mech], 0.0
+    for k in range(SC_.shape[2]):
+        for j in range(SC_.shape[1]):
+            for i in range(SC_.shape[0]):
+                if mask[i, j] == 1:
+                    assert R[i, j, k] == -7.0 and V[i, j, k] == -7.0 and D[i, j, k, isc_T - 1] == -7.0
+                    continue
+                y = SC_[i, j, k, isc_sc - 1:isc_sc - 1 + nsp]
+                v, dsp, dT, r = oracle.properties(mech, y, SC_[i, j, k, isc_T - 1], P, Lewis=Le)
+                e = max(abs(V[i, j, k] / v - 1), abs(R[i, j, k] / r - 1), abs(D[i, j, k, isc_T - 1] / dT - 1),
+                        abs(D[i, j, k, isc_z - 1] / dT - 1),
+                        np.abs(D[i, j, k, isc_sc - 1:isc_sc - 1 + nsp] / dsp - 1).max())
+                worst = max(worst, e)
+    print("%s properties: max rel err %.3e over %d cells (ghosts included)" % (wl, worst, SC_[..., 0].size))
+    assert worst < 1e-13

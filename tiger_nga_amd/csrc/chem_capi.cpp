@@ -76,6 +76,7 @@ struct tgr_chem {
   tgr_mech_info info{};
   DevMech dm{};
   void* d_tables = nullptr;
+  void* d_props = nullptr;  // transport table (properties kernel)
   unsigned int* d_queue = nullptr;
   unsigned long long* d_totals = nullptr;
   DevBuf b_ws;  // per-block solver workspace
@@ -96,6 +97,24 @@ int build_tables(tgr_chem* c) {
   c->dm.tab = (const char*)c->d_tables;
   c->dm.nsp = c->t->nsp;
   c->dm.tmid = c->t->tmid;
+  // transport table: mucoeff, kOverEps, W (64 lanes each), then the Wilke
+  // pair factors A(k,j) = (1/sqrt 8) (1 + W_k/W_j)^-1/2 and
+  // B(k,j) = (W_j/W_k)^1/4 at [j][k] (finitechem.f90:376-380)
+  const int n = c->t->nsp;
+  std::vector<double> pt(192 + 2 * (size_t)n * 64, 0.0);
+  for (int k = 0; k < n; ++k) {
+    pt[k] = c->t->mucoeff[k];
+    pt[64 + k] = c->t->koveps[k];
+    pt[128 + k] = c->t->W[k];
+  }
+  for (int k = n; k < 64; ++k) pt[128 + k] = 1.0;
+  for (int j = 0; j < n; ++j)
+    for (int k = 0; k < n; ++k) {
+      pt[192 + (size_t)j * 64 + k] = (1.0 / std::sqrt(8.0)) * std::pow(1.0 + (c->t->W[k] / c->t->W[j]), -0.5);
+      pt[192 + (size_t)n * 64 + (size_t)j * 64 + k] = std::pow(c->t->W[j] / c->t->W[k], 0.25);
+    }
+  HIP_TRY(hipMalloc(&c->d_props, pt.size() * sizeof(double)));
+  HIP_TRY(hipMemcpy(c->d_props, pt.data(), pt.size() * sizeof(double), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -255,6 +274,7 @@ int tgr_chem_create(const char* mech, double rtol, double atol, int max_steps, t
 int tgr_chem_destroy(tgr_chem* c) {
   if (!c) return 0;
   if (c->d_tables) (void)hipFree(c->d_tables);
+  if (c->d_props) (void)hipFree(c->d_props);
   if (c->d_queue) (void)hipFree(c->d_queue);
   if (c->d_totals) (void)hipFree(c->d_totals);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -359,6 +379,40 @@ int tgr_chem_prodrates_device(tgr_chem* c, int64_t n, const double* C, const dou
   if (!c || (n > 0 && (!C || !T || !P || !cdot))) return fail(TGR_E_ARG, "NULL argument");
   if (n < 0 || n > 0x7fffffff) return fail(TGR_E_ARG, "bad n");
   if (tgr_launch_prodrates(&c->dm, n, C, T, P, cdot, (hipStream_t)stream)) return fail(TGR_E_HIP, "launch");
+  return 0;
+}
+
+int tgr_chem_properties_device(tgr_chem* c, double p_thermo, const tgr_grid* g, const double* SC_, const int* mask,
+                               const double* Lewis, int isc_zmix, double* RHO_, double* VISC_, double* DIFF_,
+                               void* stream) {
+  if (!c || !SC_) return fail(TGR_E_ARG, "NULL argument");
+  int rc = check_grid(g, c->nsp);
+  if (rc) return rc;
+  if (isc_zmix < 0 || isc_zmix > g->nscalar) return fail(TGR_E_ARG, "isc_zmix out of range");
+  if (mask && (g->imino > g->imino_ || g->imaxo < g->imaxo_ || g->jmino > g->jmino_ || g->jmaxo < g->jmaxo_))
+    return fail(TGR_E_ARG, "mask does not cover the block");
+  tgr::PropsParams p{};
+  p.ptab = (const double*)c->d_props;
+  p.mtab = c->dm.tab;
+  p.tmid = c->dm.tmid;
+  p.p_thermo = p_thermo;
+  p.nx = g->imaxo_ - g->imino_ + 1;
+  p.ny = g->jmaxo_ - g->jmino_ + 1;
+  p.nz = g->kmaxo_ - g->kmino_ + 1;
+  if (p.nx <= 0 || p.ny <= 0 || p.nz <= 0) return 0;
+  p.plane = (int64_t)p.nx * p.ny * p.nz;
+  p.isc_sc0 = g->isc_sc - 1;
+  p.isc_T0 = g->isc_T - 1;
+  p.isc_z0 = isc_zmix - 1;
+  p.mask = mask;
+  p.mask_ld = g->imaxo - g->imino + 1;
+  p.mask_base = (int64_t)(g->imino_ - g->imino) + (int64_t)(g->jmino_ - g->jmino) * p.mask_ld;
+  p.lewis = Lewis;
+  p.sc = SC_;
+  p.rho = RHO_;
+  p.visc = VISC_;
+  p.diff = DIFF_;
+  if (tgr_launch_props(&p, c->neq, (hipStream_t)stream)) return fail(TGR_E_HIP, "launch");
   return 0;
 }
 

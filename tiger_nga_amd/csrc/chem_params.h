@@ -44,5 +44,23 @@ struct StepParams {
   unsigned long long* totals;    // [TGR_NCOUNT] (last: failed cells) or null
 };
 
-}  // namespace tgr
+// Transport / equation-of-state properties of a whole Fortran-layout block.
+// ptab: mucoeff[64], kOverEps[64], W[64], then A[nsp][64] and B[nsp][64]
+// (pair factors, row j, lane k).
+struct PropsParams {
+  const double* ptab;
+  const char* mtab;  // mechanism table block (thermo rows)
+  double tmid, p_thermo;
+  int nx, ny, nz;    // whole block, ghosts included
+  int64_t plane;
+  int isc_sc0, isc_T0, isc_z0;  // 0-based scalar planes; isc_z0 < 0: none
+  const int* mask;
+  int64_t mask_ld, mask_base;
+  const double* lewis;  // [nsp] or null (all 1)
+  const double* sc;
+  double* rho;   // RHO_ or null
+  double* visc;  // VISC_ or null
+  double* diff;  // DIFF_(..., nscalar) or null
+};
 
+}  // namespace tgr

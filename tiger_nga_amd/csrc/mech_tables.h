@@ -53,6 +53,8 @@ typedef struct {
   const double *eff;    /* [n_third][nsp] third-body efficiencies */
   const tgr_rxn *rxn;   /* [nrxn] */
   const tgr_nu *nu;     /* [n_nu] */
+  const double *mucoeff; /* [nsp] GETMUCOEFF, kg/(m s) */
+  const double *koveps;  /* [nsp] GETKOVEREPS, 1/K */
 } tgr_mech_tables;
 
 #ifdef __cplusplus

@@ -1,0 +1,104 @@
+// Transport and equation-of-state properties on a Fortran-layout block
+// (gfx950): the device restatement of
+//   finitechem_W           combustion/finitechem.f90:65-74
+//   finitechem_density     finitechem.f90:324-345  RHO = P_thermo W_mix / (R T)
+//   finitechem_viscosity   finitechem.f90:351-411  Wilke mixing of
+//                          eta_k = mucoeff_k sqrt(T) / omegamu(T kOverEps_k)
+//   finitechem_diffusivity finitechem.f90:414-461  lambda from the mixture
+//                          conductivities, DIFF = lambda / cp / Lewis
+//   finitechem_omegamu     finitechem.f90:466-486
+// over every cell of the block (ghosts included, as the reference loops over
+// imino_:imaxo_ ...) whose mask is not 1.  One wavefront per cell, lane k =
+// species k; the Wilke double sum runs over j with the pair factors that
+// depend on the molar masses only, (1/sqrt 8)(1 + W_k/W_j)^-1/2 and
+// (W_j/W_k)^1/4, precomputed on the host.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "chem_device.h"
+#include "chem_kernels.h"
+
+namespace tgr {
+
+constexpr double kRcst = 8314.34;  // R_cst, finitechem.f90:15 (exact double)
+
+__device__ __forceinline__ double omegamu(double t) {
+  const double m1 = 3.3530622607, m2 = 2.53272006, m3 = 2.9024238575, m4 = 0.11186138893, m5 = 0.8662326188,
+               m6 = 1.3913958626, m7 = 3.158490576, m8 = 0.18973411754, m9 = 0.00018682962894;
+  const double num = m1 + t * (m2 + t * (m3 + t * m4));
+  const double den = m5 + t * (m6 + t * (m7 + t * (m8 + t * m9)));
+  return num / den;
+}
+
+template <int NEQ>
+__global__ __launch_bounds__(256) void props_kernel(PropsParams p) {
+  constexpr int nsp = NEQ - 1;
+  constexpr TabLayout L = tab_layout<NEQ>();
+  const int lane = lane_id();
+  const bool sp = lane < nsp;
+  const TGR_GLOBAL double* pt = (const TGR_GLOBAL double*)p.ptab;
+  const double mu = sp ? pt[lane] : 0.0, koe = sp ? pt[64 + lane] : 0.0, Wk = sp ? pt[128 + lane] : 1.0;
+  const double lew = (sp && p.lewis) ? p.lewis[lane] : 1.0;
+  const int64_t ncell = (int64_t)p.nx * p.ny * p.nz;
+  const int wpb = blockDim.x >> 6;
+  for (int64_t c = (int64_t)blockIdx.x * wpb + (threadIdx.x >> 6); c < ncell; c += (int64_t)gridDim.x * wpb) {
+    const int i = (int)(c % p.nx), j = (int)((c / p.nx) % p.ny), k = (int)(c / ((int64_t)p.nx * p.ny));
+    if (p.mask && p.mask[p.mask_base + (int64_t)j * p.mask_ld + i] == 1) continue;
+    const int64_t off = c;  // the block is the whole array
+    const double Y = sp ? p.sc[(int64_t)(p.isc_sc0 + lane) * p.plane + off] : 0.0;
+    const double T = p.sc[(int64_t)p.isc_T0 * p.plane + off];
+    // finitechem_W: mass fractions clipped to [0, 1]
+    const double yt = fmin(fmax(Y, 0.0), 1.0);
+    const double Wmix = wave_sum(yt) / wave_sum(sp ? yt / Wk : 0.0);
+    if (p.rho && lane == 0) p.rho[off] = p.p_thermo * Wmix / (kRcst * T);
+    const double temp = fmin(fmax(T, 50.0), 5000.0);  // Tmin, Tmax (finitechem.f90:7-8)
+    const double eta = sp ? mu * sqrt(temp) / omegamu(temp * koe) : 1.0;
+    if (p.visc) {
+      double sum1 = 0.0;
+      for (int q = 0; q < nsp; ++q) {
+        const double etaq = bcast(eta, q), yq = bcast(Y, q), wq = pt[128 + q];
+        const double a = pt[192 + q * 64 + lane], b = pt[192 + nsp * 64 + q * 64 + lane];
+        const double t1 = sqrt(eta / etaq) * b;
+        const double phi = a * ((1.0 + t1) * (1.0 + t1));
+        if (q != lane) sum1 = sum1 + (yq * Wmix * phi / wq);
+      }
+      sum1 = sum1 + Y * Wmix / Wk;
+      const double v = wave_sum((sp && Y > 0.0) ? (Y * Wmix * eta / (Wk * sum1)) : 0.0);
+      if (lane == 0) p.visc[off] = v;
+    }
+    if (p.diff) {
+      // COMPTHERMODATA cp at the clipped temperature (table rows in HBM)
+      const TGR_GLOBAL double* th =
+          (const TGR_GLOBAL double*)(p.mtab + L.th + (temp > p.tmid ? 0 : 13 * kRow)) + lane;
+      const double cpk = sp ? th[448] * (th[512] + temp * (th[576] + temp * (th[640] + temp * (th[704] + temp * th[768]))))
+                            : 0.0;
+      const double cond = eta * (cpk + 1.25 * kRcst / Wk);
+      const double s1 = wave_sum(sp ? Y * Wmix / (cond * Wk) : 0.0);
+      const double s2 = wave_sum(sp ? Y * cond * Wmix / Wk : 0.0);
+      const double lambda = 0.5 * (s2 + (1.0 / s1));
+      const double cpm = wave_sum(Y * cpk);
+      const double D = lambda / cpm;
+      if (sp) p.diff[(int64_t)(p.isc_sc0 + lane) * p.plane + off] = D / lew;
+      if (lane == 0) p.diff[(int64_t)p.isc_T0 * p.plane + off] = D;
+      if (lane == 0 && p.isc_z0 >= 0) p.diff[(int64_t)p.isc_z0 * p.plane + off] = D;
+    }
+  }
+}
+
+}  // namespace tgr
+
+using namespace tgr;
+
+extern "C" int tgr_launch_props(const PropsParams* p, int neq, hipStream_t s) {
+  const int64_t ncell = (int64_t)p->nx * p->ny * p->nz;
+  if (ncell <= 0) return 0;
+  const int64_t need = (ncell + 3) / 4;
+  const dim3 grid((unsigned)(need < 8192 ? need : 8192)), block(256);
+  switch (neq) {
+    case 10: hipLaunchKernelGGL(props_kernel<10>, grid, block, 0, s, *p); break;
+    case 37: hipLaunchKernelGGL(props_kernel<37>, grid, block, 0, s, *p); break;
+    case 40: hipLaunchKernelGGL(props_kernel<40>, grid, block, 0, s, *p); break;
+    default: return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}

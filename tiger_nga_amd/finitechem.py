@@ -130,6 +130,8 @@ def load_library(path: str = LIB_PATH):
     lib.tgr_chem_prodrates_device.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp, vp]
     lib.tgr_chem_rhs_device.argtypes = [vp, ctypes.c_int64, ctypes.c_double, vp, vp, vp]
     lib.tgr_chem_temperature_device.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp, vp, vp]
+    lib.tgr_chem_properties_device.argtypes = [vp, ctypes.c_double, ctypes.POINTER(_Grid), vp, vp, vp, ctypes.c_int,
+                                               vp, vp, vp, vp]
     lib.tgr_chem_set_option.argtypes = [vp, ctypes.c_char_p, ctypes.c_int64]
     lib.tgr_chem_last_error.restype = ctypes.c_char_p
     _lib = lib
@@ -259,6 +261,16 @@ class FiniteChem:
         _check(self.lib, self.lib.tgr_chem_prodrates_device(self._h, n, _ptr(C), _ptr(T), _ptr(P), _ptr(out),
                                                             stream), "prodrates")
         return out
+
+    def properties(self, SC_, grid: Grid, p_thermo: float, mask=None, Lewis=None, isc_zmix: int = 0, RHO_=None,
+                   VISC_=None, DIFF_=None, stream=None):
+        """finitechem_density / finitechem_viscosity / finitechem_diffusivity
+        (finitechem.f90:324-461) on every cell of the block with mask /= 1, device
+        tensors in the step's Fortran layout; outputs that are None are skipped."""
+        g = grid._c()
+        _check(self.lib, self.lib.tgr_chem_properties_device(
+            self._h, p_thermo, ctypes.byref(g), _ptr(SC_), _ptr(mask), _ptr(Lewis), int(isc_zmix), _ptr(RHO_),
+            _ptr(VISC_), _ptr(DIFF_), stream), "properties")
 
     def temperature(self, Y, hmix, T0=None, stream=None):
         """(T[n], iters[n]) from mass fractions Y[nsp, n] and mixture enthalpy

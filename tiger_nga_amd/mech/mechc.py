@@ -11,6 +11,7 @@ them.  It parses the statements of
   ``W(r)`` and production rates ``CDOT(s)``;
 * ``COMPTHERMODATA`` (HYDROGENF.f90:514-673): pre-scaled NASA-7 polynomials;
 * ``GETMOLARMASS`` / ``GETSPECIESNAMES`` (HYDROGENF.f90:366-405);
+* ``GETMUCOEFF`` / ``GETKOVEREPS`` (HYDROGENF.f90:453-496), the transport fits;
 
 and reproduces Fortran literal kinds exactly: the reference build sets no
 default-real promotion (Makefile.in:19,23-24), so an unsuffixed real literal
@@ -619,6 +620,16 @@ def compile_mechanism(name: str, ref_root: str) -> dict:
         if re.match(r"MM\(", st, re.I):
             lhs, rhs = parse_assign(st)
             wmol[_idx(lhs.args[0], params)] = dval(fold(rhs))
+    # transport fits (GETMUCOEFF / GETKOVEREPS, e.g. HYDROGENF.f90:453-496):
+    # unsuffixed literals, so binary32 values widened to double
+    mucoeff, koveps = [None] * nsp, [None] * nsp
+    for sub, arr, var in (("GETMUCOEFF", mucoeff, "MUCOEFF"), ("GETKOVEREPS", koveps, "KOVEREPS")):
+        for no, st in subroutine_body(lines, sub):
+            if re.match(var + r"\(", st, re.I):
+                lhs, rhs = parse_assign(st)
+                arr[_idx(lhs.args[0], params)] = dval(fold(rhs))
+    if None in mucoeff or None in koveps:
+        raise MechError("incomplete transport data")
     nreac_decl = None
     for no, st in subroutine_body(lines, "GETNREACTIONS"):
         m = re.match(r"NREACTIONS\s*=\s*(\d+)", st, re.I)
@@ -701,7 +712,7 @@ def compile_mechanism(name: str, ref_root: str) -> dict:
 
     return {
         "name": name, "source": f90, "header": h,
-        "species": names, "W": wmol, "nslot": nslot, "n_third": nm,
+        "species": names, "W": wmol, "mucoeff": mucoeff, "koveps": koveps, "nslot": nslot, "n_third": nm,
         "third_body": eff, "tmid": tmid,
         "thermo_hi": [thermo["hi"][s] for s in range(nsp)],
         "thermo_lo": [thermo["lo"][s] for s in range(nsp)],
@@ -763,6 +774,8 @@ def emit_c_header(mech: dict) -> str:
     w("")
     w("static const char *const %s_species[%d] = {%s};" % (n, nsp, ", ".join('"%s"' % s for s in mech["species"])))
     w("static const double %s_W[%d] = {%s};" % (n, nsp, ", ".join(_c(x) for x in mech["W"])))
+    w("static const double %s_mucoeff[%d] = {%s};" % (n, nsp, ", ".join(_c(x) for x in mech["mucoeff"])))
+    w("static const double %s_koveps[%d] = {%s};" % (n, nsp, ", ".join(_c(x) for x in mech["koveps"])))
     # thermo: [range][species][13]: a_h, c1..c5, c6, a_cp, d1..d5
     rows = []
     for blk in ("thermo_hi", "thermo_lo"):
@@ -812,7 +825,8 @@ def emit_c_header(mech: dict) -> str:
     w("")
     w("static const tgr_mech_tables %s_tables = {" % n)
     w('  "%s", %d, %d, %d, %d, %d, %s,' % (n, nsp, len(rx), mech["nslot"], nm, len(nus), _c(mech["tmid"])))
-    w("  %s_species, %s_W, &%s_thermo[0][0], &%s_eff[0][0], %s_rxn, %s_nu};" % (n, n, n, n, n, n))
+    w("  %s_species, %s_W, &%s_thermo[0][0], &%s_eff[0][0], %s_rxn, %s_nu, %s_mucoeff, %s_koveps};"
+      % (n, n, n, n, n, n, n, n))
     return "\n".join(L) + "\n"
 
 
