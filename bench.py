@@ -35,6 +35,9 @@ def parse():
     ap.add_argument("--cpu-cells", type=int, default=131072)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all usable host cores, at most 16")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel", default="step", choices=["step", "properties"],
+                    help="step: the chemistry step (the headline); properties: density / viscosity / "
+                         "diffusivities of the same block")
     return ap.parse_args()
 
 
@@ -79,6 +82,38 @@ def traffic_record(workload):
     return None
 
 
+def bench_properties(a, torch, fc, g, S_d, R_d, P, n, nx, mech, sp):
+    """Density, viscosity and diffusivities of every cell (finitechem.f90:324-461)."""
+    from tiger_nga_amd import flops
+    V_d = torch.empty_like(R_d)
+    D_d = torch.empty_like(S_d)
+    for _ in range(max(1, a.warmup)):
+        fc.properties(S_d, g, P, RHO_=R_d, VISC_=V_d, DIFF_=D_d, stream=sp)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(a.steps):
+        fc.properties(S_d, g, P, RHO_=R_d, VISC_=V_d, DIFF_=D_d, stream=sp)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ks = e0.elapsed_time(e1) / 1e3 / a.steps
+    F = flops.properties_flops(fc.nsp) * n
+    B = 8.0 * n * ((fc.nsp + 1) + (fc.nsp + 3))
+    out = {"metric": "transport/EOS property cell-updates/s, 1 GPU", "value": n * a.steps / el, "unit": "cells/s",
+           "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": el / a.steps * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "%s density + Wilke viscosity + diffusivities, %d^3 = %d cells" % (mech, nx, n)},
+           "roofline": {"bound": "mfma", "compute_unit": "FP64 vector ALU", "achieved": F / ks / 1e12,
+                        "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": F / ks / PEAK_FP64, "traffic": None,
+                        "kernel_ms": ks * 1e3, "flops_per_step": F, "algorithmic_bytes_per_step": B,
+                        "hbm_frac_at_algorithmic_bytes": B / ks / PEAK_HBM}}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     a = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -111,6 +146,8 @@ def main():
     fc = FiniteChem(mech, 1e-6, 1e-12, 10000)
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
+    if a.kernel == "properties":
+        return bench_properties(a, torch, fc, g, S0_d, R_d, P, n, nx, mech, sp)
 
     stats = None
     for _ in range(max(1, a.warmup)):

@@ -54,13 +54,17 @@ __global__ __launch_bounds__(256) void props_kernel(PropsParams p) {
     const double temp = fmin(fmax(T, 50.0), 5000.0);  // Tmin, Tmax (finitechem.f90:7-8)
     const double eta = sp ? mu * sqrt(temp) / omegamu(temp * koe) : 1.0;
     if (p.visc) {
+      // per pair: (eta_k/eta_j)^1/2 as sqrt(eta_k) / sqrt(eta_j) and the
+      // weight Y_j W_mix / W_j formed once per species (rounding-level
+      // differences from the reference's expression order)
+      const double se = sqrt(eta), ise = 1.0 / se, yw = sp ? Y * Wmix / Wk : 0.0;
       double sum1 = 0.0;
       for (int q = 0; q < nsp; ++q) {
-        const double etaq = bcast(eta, q), yq = bcast(Y, q), wq = pt[128 + q];
+        const double iseq = bcast(ise, q), ywq = bcast(yw, q);
         const double a = pt[192 + q * 64 + lane], b = pt[192 + nsp * 64 + q * 64 + lane];
-        const double t1 = sqrt(eta / etaq) * b;
+        const double t1 = se * iseq * b;
         const double phi = a * ((1.0 + t1) * (1.0 + t1));
-        if (q != lane) sum1 = sum1 + (yq * Wmix * phi / wq);
+        sum1 = q != lane ? fma(ywq, phi, sum1) : sum1;
       }
       sum1 = sum1 + Y * Wmix / Wk;
       const double v = wave_sum((sp && Y > 0.0) ? (Y * Wmix * eta / (Wk * sum1)) : 0.0);

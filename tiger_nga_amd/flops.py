@@ -57,3 +57,12 @@ def step_bytes(info: dict, cells: int) -> float:
     three source terms per cell, 8 bytes each."""
     nsp = info["nsp"]
     return 8.0 * cells * ((nsp + 2) + (nsp + 1) + 4)
+
+
+def properties_flops(nsp: int) -> float:
+    """Algorithmic flops of one cell of the properties kernel (same convention;
+    sqrt = 1): W_mix (3 nsp), species viscosities (omegamu 16 + 3 per species),
+    Wilke double sum (10 per pair: ratio, sqrt, two products, square, the
+    weighted term and its accumulation), conductivity and the two mixture sums
+    (12 per species), density and outputs (5 + nsp)."""
+    return 3 * nsp + 19 * nsp + 10 * nsp * nsp + 12 * nsp + 5 + nsp
