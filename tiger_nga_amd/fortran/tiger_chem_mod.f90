@@ -32,6 +32,7 @@ module tiger_chem
 
   public :: tgr_chem_create, tgr_chem_destroy, tgr_chem_source_chemistry, tgr_chem_source_chemistry_device
   public :: tgr_chem_molar_mass, tgr_chem_species_name, tgr_chem_set_option, tgr_chem_last_error
+  public :: tgr_chem_properties_device, tgr_chem_temperature_device
   public :: tiger_chem_init, tiger_chem_final, tiger_chem_source_chemistry, tiger_chem_message
   public :: tiger_chem_ctx
 
@@ -88,6 +89,28 @@ module tiger_chem
        type(tgr_grid), intent(in) :: g
        type(c_ptr), value :: rho, sc, mask, hr, chemsrc, stream, stats
      end function tgr_chem_source_chemistry_device
+
+     ! finitechem_density / finitechem_viscosity / finitechem_diffusivity on
+     ! device arrays (rho, visc, diff, lewis, mask may be c_null_ptr)
+     integer(c_int) function tgr_chem_properties_device(ctx, p_thermo, g, sc, mask, lewis, isc_zmix, rho, visc, &
+          diff, stream) bind(C, name="tgr_chem_properties_device")
+       import :: c_int, c_ptr, c_double, tgr_grid
+       type(c_ptr), value :: ctx
+       real(c_double), value :: p_thermo
+       type(tgr_grid), intent(in) :: g
+       type(c_ptr), value :: sc, mask, lewis
+       integer(c_int), value :: isc_zmix
+       type(c_ptr), value :: rho, visc, diff, stream
+     end function tgr_chem_properties_device
+
+     ! temperature from mixture enthalpy, device arrays
+     integer(c_int) function tgr_chem_temperature_device(ctx, n, Y, hmix, T0, T, iters, stream) &
+          bind(C, name="tgr_chem_temperature_device")
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), value :: n
+       type(c_ptr), value :: Y, hmix, T0, T, iters, stream
+     end function tgr_chem_temperature_device
 
      type(c_ptr) function tgr_chem_last_error() bind(C, name="tgr_chem_last_error")
        import :: c_ptr
