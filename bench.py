@@ -38,6 +38,12 @@ def parse():
     ap.add_argument("--kernel", default="step", choices=["step", "properties"],
                     help="step: the chemistry step (the headline); properties: density / viscosity / "
                          "diffusivities of the same block")
+    ap.add_argument("--data-file", default=None,
+                    help="step the fields of an NGA data file instead of the synthetic workload "
+                         "(needs --mech; no CPU baseline)")
+    ap.add_argument("--mech", default=None, choices=["h2", "gri30", "dme39"], help="mechanism of --data-file")
+    ap.add_argument("--pressure", type=float, default=101325.0, help="P_thermo for --data-file (Pa)")
+    ap.add_argument("--dt", type=float, default=0.0, help="chemistry dt for --data-file (0: the file's dt)")
     return ap.parse_args()
 
 
@@ -82,7 +88,7 @@ def traffic_record(workload):
     return None
 
 
-def bench_properties(a, torch, fc, g, S_d, R_d, P, n, nx, mech, sp):
+def bench_properties(a, torch, fc, g, S_d, R_d, P, n, cells_label, mech, sp):
     """Density, viscosity and diffusivities of every cell (finitechem.f90:324-461)."""
     from tiger_nga_amd import flops
     V_d = torch.empty_like(R_d)
@@ -105,7 +111,7 @@ def bench_properties(a, torch, fc, g, S_d, R_d, P, n, nx, mech, sp):
     out = {"metric": "transport/EOS property cell-updates/s, 1 GPU", "value": n * a.steps / el, "unit": "cells/s",
            "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": el / a.steps * 1e3,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": "%s density + Wilke viscosity + diffusivities, %d^3 = %d cells" % (mech, nx, n)},
+           "config": {"workload": "%s density + Wilke viscosity + diffusivities, %s" % (mech, cells_label)},
            "roofline": {"bound": "mfma", "compute_unit": "FP64 vector ALU", "achieved": F / ks / 1e12,
                         "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": F / ks / PEAK_FP64, "traffic": None,
                         "kernel_ms": ks * 1e3, "flops_per_step": F, "algorithmic_bytes_per_step": B,
@@ -129,25 +135,45 @@ def main():
 
     dev = "cuda:0"
     torch.cuda.set_device(0)
-    mech, Y, T, rho, P, dt = inputs.states(a.workload, seed=0)
-    n = Y.shape[0]
-    nx = int(round(n ** (1.0 / 3.0)))
-    assert nx ** 3 == n
-    RHO_, SC_, isc_sc, isc_T = inputs.fortran_fields(Y, T, rho, nx, nx, nx)
-    g = Grid.block(nx, nx, nx, SC_.shape[3], isc_sc, isc_T)
-    R_d = fortran_empty(torch, RHO_.shape, dev)
-    S0_d = fortran_empty(torch, SC_.shape, dev)
-    S_d = fortran_empty(torch, SC_.shape, dev)
-    HR_d = fortran_empty(torch, RHO_.shape, dev)
-    CS_d = fortran_empty(torch, RHO_.shape + (3,), dev)
-    R_d.copy_(torch.from_numpy(RHO_))
-    S0_d.copy_(torch.from_numpy(SC_))
-    del RHO_, SC_, Y, T, rho
+    if a.data_file:
+        from tiger_nga_amd import ngadata
+        from tiger_nga_amd.mech import meta
+        if not a.mech:
+            print("--data-file needs --mech", file=sys.stderr)
+            return 2
+        df = ngadata.read(a.data_file)
+        mech, P = a.mech, a.pressure
+        dt = a.dt or df.dt
+        if not dt > 0.0:
+            print("the data file holds dt = %g: give --dt" % df.dt, file=sys.stderr)
+            return 2
+        R_d, S0_d, g = ngadata.chemistry_block(df, meta.species(mech), device=dev)
+        n = df.nx * df.ny * df.nz
+        cells_label = "%dx%dx%d = %d cells" % (df.nx, df.ny, df.nz, n)
+        workload = "data file %s" % os.path.basename(a.data_file)
+        a.no_cpu = True
+    else:
+        mech, Y, T, rho, P, dt = inputs.states(a.workload, seed=0)
+        n = Y.shape[0]
+        nx = int(round(n ** (1.0 / 3.0)))
+        assert nx ** 3 == n
+        RHO_, SC_, isc_sc, isc_T = inputs.fortran_fields(Y, T, rho, nx, nx, nx)
+        g = Grid.block(nx, nx, nx, SC_.shape[3], isc_sc, isc_T)
+        R_d = fortran_empty(torch, RHO_.shape, dev)
+        S0_d = fortran_empty(torch, SC_.shape, dev)
+        R_d.copy_(torch.from_numpy(RHO_))
+        S0_d.copy_(torch.from_numpy(SC_))
+        del RHO_, SC_, Y, T, rho
+        cells_label = "%d^3 = %d cells" % (nx, n)
+        workload = a.workload
+    S_d = fortran_empty(torch, tuple(S0_d.shape), dev)
+    HR_d = fortran_empty(torch, tuple(R_d.shape), dev)
+    CS_d = fortran_empty(torch, tuple(R_d.shape) + (3,), dev)
     fc = FiniteChem(mech, 1e-6, 1e-12, 10000)
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
     if a.kernel == "properties":
-        return bench_properties(a, torch, fc, g, S0_d, R_d, P, n, nx, mech, sp)
+        return bench_properties(a, torch, fc, g, S0_d, R_d, P, n, cells_label, mech, sp)
 
     stats = None
     for _ in range(max(1, a.warmup)):
@@ -184,9 +210,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic seeded states (no case data ships with the reference)",
-        "config": {"workload": "%s plain FP64 step, %d^3 = %d cells, dt %.1e s, P %.0f Pa, rtol 1e-6 atol 1e-12"
-                               % (a.workload, nx, n, dt, P), "mechanism": mech, "path": "plain_fp64",
+        "data": ("fields of %s" % a.data_file) if a.data_file else
+                "synthetic seeded states (no case data ships with the reference)",
+        "config": {"workload": "%s plain FP64 step, %s, dt %.1e s, P %.0f Pa, rtol 1e-6 atol 1e-12"
+                               % (workload, cells_label, dt, P), "mechanism": mech, "path": "plain_fp64",
                    "species": info["nsp"], "reactions": info["nrxn"], "rate_slots": info["nslot"]},
         "roofline": {"bound": "mfma", "compute_unit": "FP64 vector ALU (MI355X FP64 vector peak = FP64 matrix "
                                                        "peak = 78.6 TFLOP/s)",
@@ -197,7 +224,7 @@ def main():
         "solver": {k: stats[k] for k in ("nst", "nfe", "nfe_dq", "nje", "nsetups", "nni", "netf", "ncfn", "nlu",
                                          "failed")},
     }
-    tr = traffic_record(a.workload)
+    tr = None if a.data_file else traffic_record(a.workload)
     if tr:
         out["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
         out["roofline"]["traffic_source"] = tr["source"]
