@@ -8,15 +8,20 @@ ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
 
 
 def declared():
-    txt = open(os.path.join(ROOT, "include", "tiger_chem.h")).read()
-    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(tgr_chem_\w+)\s*\(", txt)))
+    names = set()
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if h.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", h)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names |= set(re.findall(r"\b(tgr_\w+)\s*\(", txt))
+    return sorted(names)
 
 
 def test_header_declares_the_entry_points():
     names = declared()
     for must in ("tgr_chem_create", "tgr_chem_destroy", "tgr_chem_source_chemistry",
-                 "tgr_chem_source_chemistry_device", "tgr_chem_last_error"):
+                 "tgr_chem_source_chemistry_device", "tgr_chem_last_error", "tgr_mlp_train_step",
+                 "tgr_mlp_forward"):
         assert must in names
 
 
