@@ -131,6 +131,64 @@ int tgr_chem_properties_device(tgr_chem *ctx, double p_thermo, const tgr_grid *g
 int tgr_chem_temperature_device(tgr_chem *ctx, int64_t n, const double *Y, const double *hmix, const double *T0,
                                 double *T, int *iters, void *stream);
 
+/* ---- surrogate-guided step (no counterpart in the reference) -------------
+ * The learned part: a perceptron (include/tiger_mlp.h) trained on (state, dt)
+ * -> increment pairs that the plain FP64 step generates predicts every cell's
+ * state at the stage times of a one-step, L-stable SDIRK solve (two stages,
+ * gamma = 1 - 1/sqrt 2, `surr_nsub` sub-steps, one difference-quotient
+ * Jacobian and one LU per cell); FP64 Newton iterations start from those
+ * predictions.  A cell is accepted when every Newton solve converges, the
+ * embedded error estimate (filtered by (I - gamma h J)^-1) stays within
+ * |dY_k| <= surr_tol_y and |dT| <= surr_tol_t, and the temperature implied by
+ * conservation of the mixture enthalpy agrees with the solved one within
+ * surr_tol_t; the species are projected back onto exact conservation of each
+ * element's mass and the enthalpy-consistent temperature is written.  Rejected
+ * cells are integrated by the plain FP64 step on the same call, so every cell
+ * ends either accepted or exactly as tgr_chem_source_chemistry_device leaves
+ * it.  Adaptive use: while the accepted fraction of the last guided call is
+ * below "surr_min_accept" (default 0.15), calls run the plain step only,
+ * except one guided probe every "surr_probe" calls (default 8; 0 = always
+ * guided).  With surr_probe > 0 a guided call reads the accepted count back
+ * and is therefore synchronous. */
+typedef struct tgr_mlp tgr_mlp;
+
+typedef struct {
+  int64_t accepted, rejected;     /* cells accepted by the guided solve / handed to the plain step */
+  int64_t newton_iters, rhs_evals; /* of the guided solve (Jacobian columns included) */
+  int64_t rej_newton, rej_error, rej_enthalpy; /* why cells were rejected */
+  double surrogate_ms, fallback_ms; /* device time of features + network + guided solve, of the plain step */
+  int guided;                     /* 1: the guided solve ran; 0: the adaptive policy sent the call to the plain step */
+} tgr_surrogate_stats;
+
+/* Attach a trained network (borrowed; NULL detaches).  Inputs: nsp + 2
+ * (log10 max(Y_k, 1e-30), T, log10 dt), each scaled as (x - in_shift) *
+ * in_scale; outputs: nsp + 1, the increments of Y_k and T over dt divided by
+ * out_scale. */
+int tgr_chem_set_surrogate(tgr_chem *ctx, tgr_mlp *net, const float *in_shift, const float *in_scale,
+                           const double *out_scale);
+
+/* Network inputs of the interior cells of a block (i fastest) as bfloat16 bit
+ * patterns X[rows][in_pad], rows >= cells and a multiple of 64 (the rows past
+ * the cells are zero): the features the surrogate step feeds the network. */
+int tgr_chem_surrogate_features_device(tgr_chem *ctx, double dt, const tgr_grid *g, const double *SC_, uint16_t *X,
+                                       int64_t rows, void *stream);
+
+/* finitechem_source_chemistry (finitechem.f90:962-1071) on DEVICE arrays by
+ * the surrogate-guided path described above.  stats: the whole call (counters
+ * of the plain step on the rejected cells, kernel_ms = total); sstats: the
+ * guided solve.  Either may be NULL (then the call only enqueues). */
+int tgr_chem_source_chemistry_surrogate_device(tgr_chem *ctx, double dt, double p_thermo, double p_thermo_old,
+                                               const tgr_grid *g, double *RHO_, double *SC_, const int *mask,
+                                               double *HR_gp, double *chemsrc_gp, void *stream, tgr_stats *stats,
+                                               tgr_surrogate_stats *sstats);
+
+/* Real-valued knobs of the surrogate step: "surr_tol_y" (default 1e-6),
+ * "surr_tol_t" (K, default 1e-2), "surr_newton_tol" (Newton stops when the
+ * correction is below this fraction of the acceptance tolerance, default 0.1),
+ * "surr_grade" (ratio of consecutive sub-step sizes, default 1),
+ * "surr_min_accept" (adaptive-use threshold, default 0.15). */
+int tgr_chem_set_option_real(tgr_chem *ctx, const char *key, double value);
+
 /* Tuning knobs (no counterpart in the reference):
  *   "grid"        size of the persistent grid in workgroups of 8 waves (0: occupancy x CUs);
  *                 every wave pulls cells
@@ -138,7 +196,13 @@ int tgr_chem_temperature_device(tgr_chem *ctx, int64_t n, const double *Y, const
  *   "rhs_mapping" tgr_chem_rhs_device: 1 (default) one state per lane (generated
  *                 straight-line code of the mechanism), 0 one state per wavefront
  *   "max_order"   BDF maximum order 1..5 (CVODE default 5)
- *   "max_steps"   MAX_NSTEPS */
+ *   "max_steps"   MAX_NSTEPS
+ *   "surr_nsub"   sub-steps of the surrogate-guided solve, 1..8 (default 4)
+ *   "surr_guess"  1 (default): Newton from the network's prediction; 0: from
+ *                 the sub-step's initial state (the same solve without the network)
+ *   "surr_newton_max"  Newton iterations per stage before the cell is rejected (default 8)
+ *   "surr_probe"  adaptive use: a guided probe every this many calls while the
+ *                 guided solve is not paying (default 8; 0 = always guided) */
 int tgr_chem_set_option(tgr_chem *ctx, const char *key, int64_t value);
 
 /* Message of the last failing call on this thread. */

@@ -40,6 +40,9 @@ int tgr_mlp_destroy(tgr_mlp *h);
 /* Length of the parameter vector and padded input / output widths. */
 int tgr_mlp_layout(const tgr_mlp *h, int64_t *nparam, int *in_pad, int *out_pad);
 
+/* The shape given to tgr_mlp_create. */
+int tgr_mlp_shape(const tgr_mlp *h, int *nin, int *nout, int *hidden, int *n_hidden_layers);
+
 /* Copy the FP32 parameters from / to host memory (set also resets Adam). */
 int tgr_mlp_set_params(tgr_mlp *h, const float *host);
 int tgr_mlp_get_params(const tgr_mlp *h, float *host);

@@ -1,0 +1,92 @@
+"""The surrogate-guided step through the C ABI, with the committed trained
+networks (tiger_nga_amd/models/): every cell ends either accepted by the
+guided solve (then within the acceptance bound of the plain FP64 result and
+element-conserving) or bit-identical to the plain step, which integrated it;
+and what the trained network gains over the same solve without it."""
+import os
+
+import numpy as np
+import pytest
+
+from tiger_nga_amd.mech import meta
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    assert torch.cuda.is_available(), "GPU test without a HIP device"
+    return torch
+
+
+def _model(mech):
+    from tiger_nga_amd import surrogate
+    path = surrogate.model_path(mech)
+    assert os.path.exists(path), "trained network missing: %s" % path
+    return surrogate.Surrogate.load(path)
+
+
+@pytest.mark.parametrize("mech,workload,n,relax", [
+    ("h2", "h2", 16 ** 3, 0), ("h2", "h2", 16 ** 3, 1), ("gri30", "gri30", 16 ** 3, 0), ("gri30", "gri30", 16 ** 3, 1)])
+def test_cells_accepted_within_bound_or_plain(torch_dev, mech, workload, n, relax):
+    torch = torch_dev
+    from tiger_nga_amd import surrogate
+    from tiger_nga_amd.finitechem import FiniteChem
+    fc = FiniteChem(mech, 1e-6, 1e-12)
+    R_d, S0, g, P = surrogate.block_states(fc, workload, n, 3, relax)
+    dt = {"h2": 1e-6, "gri30": 5e-7}[mech]
+    nsp = fc.nsp
+    Sp = S0.clone()
+    HRp = torch.zeros_like(R_d)
+    stp = fc.source_chemistry(R_d, Sp, g, dt, P, P, HR_gp=HRp)
+    sur = _model(mech)
+    sur.attach(fc)
+    Ss = S0.clone()
+    HRs = torch.zeros_like(R_d)
+    st, ss = fc.source_chemistry_surrogate(R_d, Ss, g, dt, P, P, HR_gp=HRs)
+    assert ss["accepted"] + ss["rejected"] == n == st["cells"]
+    assert st["failed"] == 0 and stp["failed"] == 0
+    same = (Ss == Sp).all(dim=3)
+    assert int((~same).sum()) <= ss["accepted"]
+    # accepted cells: within the acceptance bound of the plain FP64 step
+    dY = (Ss[..., :nsp] - Sp[..., :nsp]).abs().amax(dim=3)
+    dT = (Ss[..., nsp] - Sp[..., nsp]).abs()
+    assert float(dY.max()) <= 1e-6 and float(dT.max()) <= 2e-2
+    # element conservation of every cell (Y in mass fractions)
+    Et = torch.from_numpy(np.ascontiguousarray((meta.element_matrix(mech) / meta.molar_mass(mech)[None, :]).T)).cuda()
+    el0 = S0[..., :nsp].contiguous() @ Et
+    el1 = Ss[..., :nsp].contiguous() @ Et
+    elp = Sp[..., :nsp].contiguous() @ Et
+    cons = float(((el1 - el0).abs() / el0.abs().clamp_min(1e-30)).max())
+    cons_plain = float(((elp - el0).abs() / el0.abs().clamp_min(1e-30)).max())
+    print("\nelement conservation: guided %.2e, plain %.2e (max relative change)" % (cons, cons_plain))
+    assert cons <= max(1e-12, 2.0 * cons_plain)
+    print("\n%s relax %d: accepted %d / %d, rejected (Newton %d, error %d, enthalpy %d); max|dY| %.2e max|dT| %.2e K"
+          % (mech, relax, ss["accepted"], n, ss["rej_newton"], ss["rej_error"], ss["rej_enthalpy"], float(dY.max()),
+             float(dT.max())))
+
+
+def test_network_guess_gain(torch_dev):
+    """Newton iterations of the guided solve (stages converged to 1e-8 of the
+    acceptance tolerance, acceptance off) with the network's guesses against
+    the same solve started from the previous stage value, on relaxed GRI-3.0
+    states: the trained network saves iterations."""
+    from tiger_nga_amd import surrogate
+    from tiger_nga_amd.finitechem import FiniteChem
+    fc = FiniteChem("gri30", 1e-6, 1e-12)
+    R_d, S0, g, P = surrogate.block_states(fc, "gri30", 32 ** 3, 4, 1)
+    _model("gri30").attach(fc)
+    fc.set_option("surr_newton_max", 20)
+    fc.set_option("surr_tol_y", 1.0)
+    fc.set_option("surr_tol_t", 1e4)
+    fc.set_option("surr_newton_tol", 1e-8)
+    it = {}
+    for guess in (1, 0):
+        fc.set_option("surr_guess", guess)
+        S = S0.clone()
+        _, ss = fc.source_chemistry_surrogate(R_d, S, g, 5e-7, P, P)
+        it[guess] = ss["newton_iters"] / (32 ** 3)
+    print("\nNewton iterations per cell: with the network %.2f, without %.2f (%.1f %% fewer)"
+          % (it[1], it[0], 100.0 * (1.0 - it[1] / it[0])))
+    assert it[1] < it[0]

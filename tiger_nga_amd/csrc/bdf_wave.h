@@ -670,6 +670,106 @@ struct Bdf {
       }
     }
   }
+  // ---- guided solve (surrogate-guided path) -------------------------------
+  // nsub steps of sizes hs[0..nsub) (sum dt) of the three-stage, L-stable,
+  // stiffly accurate SDIRK method of order 3 (Alexander 1977; gamma the root
+  // of x^3 - 3x^2 + 3x/2 - 1/6 in (1/6, 1/2), c = (gamma, (1+gamma)/2, 1),
+  // a21 = (1-gamma)/2, b = (-(6g^2-16g+1)/4, (6g^2-20g+5)/4, gamma)) with one
+  // difference-quotient Jacobian at the initial state and one LU of
+  // I - gamma h J per sub-step.  Stage values are solved by simplified Newton
+  // starting, with use_guess, from the previous stage value moved by the
+  // network's change guess(i, j) - guess(previous stage) (sub-step i, stage j,
+  // packed variables), else from the previous stage value.  Error estimate per
+  // sub-step: the embedded second-order solution (weights bh = (1 - bh2, bh2,
+  // 0), bh2 = (1/2 - gamma) / (c2 - gamma)) compared with the third-order one,
+  // filtered by (I - gamma h J)^-1, in units of the acceptance tolerance
+  // (wacc = 1 / tolerance of this lane's component).  Returns 0 (accepted,
+  // yv = solution), 1 (a Newton solve did not converge in nmax iterations or
+  // the matrix is singular), 2 (error estimate above 1).
+  static constexpr double kSdG = 0.43586652150845899942;
+  static constexpr double kSdC2 = 0.5 * (1.0 + kSdG);
+  static constexpr double kSdA21 = 0.5 * (1.0 - kSdG);
+  static constexpr double kSdB1 = -(6.0 * kSdG * kSdG - 16.0 * kSdG + 1.0) / 4.0;
+  static constexpr double kSdB2 = (6.0 * kSdG * kSdG - 20.0 * kSdG + 5.0) / 4.0;
+  static constexpr double kSdBh2 = (0.5 - kSdG) / (kSdC2 - kSdG);
+  static constexpr double kSdBh1 = 1.0 - kSdBh2;
+
+  __device__ __forceinline__ bool sdirk_newton(double& z, double base, double gh, double wacc, int nmax, double ntol,
+                                               int& nit) {
+    for (int it = 0; it < nmax; ++it) {
+      const double fz = f(z);
+      c_nfe++;
+      const double r = act ? z - base - gh * fz : 0.0;
+      const double d = getrs(-r);
+      nit++;
+      if (act) z += d;
+      const double dn = wave_max(act ? fabs(d) * wacc : 0.0);
+      if (dn <= ntol) return true;
+      if (!isfinite(dn)) return false;
+    }
+    return false;
+  }
+
+  template <class Guess>
+  __device__ __forceinline__ int sdirk_guided(double& yv, int nsub, const double* hs, const Guess& guess,
+                                              bool use_guess, double wacc, int nmax, double ntol, int& nit) {
+    c_nfe = c_nfe_dq = c_nje = c_nsetups = c_nni = c_netf = c_ncfn = c_nlu = 0;
+    nst = 0;
+    h = hs[0];
+    const double y0 = act ? yv : 0.0;
+    set_ewt(y0);
+    const double f0 = f(y0);
+    c_nfe++;
+    dq_jac(y0, f0);
+    c_nje++;
+    double yi = y0;
+    // Newton starts from the previous stage's converged value plus the
+    // network's predicted change between the two stage times (its bias,
+    // smooth in time, cancels); gp: the network's value at the previous stage
+    double gp = y0;
+    auto start = [&](int i, int j, double zprev) -> double {
+      if (!use_guess) return zprev;
+      const double gc = guess(i, j);
+      const double z0 = zprev + (gc - gp);
+      gp = gc;
+      return z0;
+    };
+    for (int i = 0; i < nsub; ++i) {
+      const double hh = hs[i];
+      const double gh = kSdG * hh;
+      {
+        const TGR_GLOBAL double* jr = fresh(jw) + lane;
+#pragma unroll
+        for (int c = 0; c < NEQ; ++c) {
+          double v = -gh * jr[c * kWave];
+          if (c == lane) v += 1.0;
+          M[c] = v;
+        }
+      }
+      c_nlu++;
+      if (getrf()) return 1;
+      const double rg = 1.0 / gh;
+      double z = start(i, 0, yi);
+      if (!sdirk_newton(z, yi, gh, wacc, nmax, ntol, nit)) return 1;
+      const double k1 = act ? (z - yi) * rg : 0.0;
+      const double base2 = act ? yi + kSdA21 * hh * k1 : 0.0;
+      z = start(i, 1, z);
+      if (!sdirk_newton(z, base2, gh, wacc, nmax, ntol, nit)) return 1;
+      const double k2 = act ? (z - base2) * rg : 0.0;
+      const double base3 = act ? yi + hh * (kSdB1 * k1 + kSdB2 * k2) : 0.0;
+      z = start(i, 2, z);
+      if (!sdirk_newton(z, base3, gh, wacc, nmax, ntol, nit)) return 1;
+      const double k3 = act ? (z - base3) * rg : 0.0;
+      const double e =
+          getrs(act ? hh * ((kSdB1 - kSdBh1) * k1 + (kSdB2 - kSdBh2) * k2 + kSdG * k3) : 0.0);
+      const double en = wave_max(act ? fabs(e) * wacc : 0.0);
+      nst++;
+      if (!(en <= 1.0)) return 2;
+      yi = z;
+    }
+    yv = yi;
+    return 0;
+  }
 };
 
 // One cell of finitechem_source_chemistry (finitechem.f90:996-1058): pack

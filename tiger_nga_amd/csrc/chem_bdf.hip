@@ -36,11 +36,13 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepParams p) {
   const int lane = lane_id();
   const int w = wave_in_block();
   double* ws = p.ws + ((int64_t)blockIdx.x * kWavesPerBlock + w) * NEQ * kWave;
+  const int64_t ntake = p.cell_list ? (int64_t)*p.nlist : p.ncell;
   for (;;) {
     unsigned int c = 0;
     if (lane == 0) c = atomicAdd(p.queue, 1u);
-    const int64_t cell = (int64_t)(unsigned int)__builtin_amdgcn_readfirstlane((int)c);
-    if (cell >= p.ncell) break;
+    const int64_t idx = (int64_t)(unsigned int)__builtin_amdgcn_readfirstlane((int)c);
+    if (idx >= ntake) break;
+    const int64_t cell = p.cell_list ? (int64_t)p.cell_list[idx] : idx;
     int cnt[TGR_NCOUNT];
     if (!process_cell<NEQ>(p, m, (tchar*)tab_s, (TGR_LDS RhsLds<NEQ>*)&rl[w], cell,
                            *(TGR_LDS double(*)[kQmaxAbs + 2][kWave]) & zn_s[w], ws, cnt))

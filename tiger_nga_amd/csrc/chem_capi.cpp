@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/tiger_chem.h"
+#include "../../include/tiger_mlp.h"
 #include "chem_kernels.h"
 #include "tables_host.h"
 #include "mech_dme39.h"
@@ -18,6 +19,7 @@
 
 using tgr::DevMech;
 using tgr::StepParams;
+using tgr::SurrParams;
 
 namespace {
 
@@ -83,6 +85,26 @@ struct tgr_chem {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // staging for the host-array entry point
   DevBuf b_rho, b_sc, b_mask, b_hr, b_chem;
+  // surrogate-guided step: network (borrowed), feature / output scaling,
+  // network inputs and outputs, rejected-cell list and counters
+  tgr_mlp* net = nullptr;
+  int net_in_pad = 0, net_out_pad = 0;
+  double* d_elem = nullptr;  // [TGR_NELEM][64] element atoms / W (surrogate projection)
+  float* d_in_shift = nullptr;
+  float* d_in_scale = nullptr;
+  double* d_out_scale = nullptr;
+  std::vector<float> h_in_shift, h_in_scale;
+  DevBuf b_x, b_guess, b_rej;
+  unsigned int* d_rej_count = nullptr;
+  unsigned long long* d_stotals = nullptr;
+  int surr_nsub = 4, surr_guess = 1, surr_newton_max = 8;
+  // adaptive use: while the accepted fraction of the last guided call is
+  // below surr_min_accept, calls go to the plain step, with a guided probe
+  // every surr_probe calls (0: always guided)
+  int surr_probe = 8, surr_skipped = 0;
+  double surr_min_accept = 0.15, surr_last_accept = 1.0;
+  double surr_newton_tol = 0.1, surr_tol_y = 1.0e-6, surr_tol_t = 1.0e-2, surr_grade = 1.0;
+  hipEvent_t ev2 = nullptr;
 };
 
 namespace {
@@ -115,6 +137,11 @@ int build_tables(tgr_chem* c) {
     }
   HIP_TRY(hipMalloc(&c->d_props, pt.size() * sizeof(double)));
   HIP_TRY(hipMemcpy(c->d_props, pt.data(), pt.size() * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<double> el((size_t)TGR_NELEM * 64, 0.0);
+  for (int e = 0; e < TGR_NELEM; ++e)
+    for (int k = 0; k < n; ++k) el[(size_t)e * 64 + k] = c->t->elem[(size_t)e * n + k] / c->t->W[k];
+  HIP_TRY(hipMalloc(&c->d_elem, el.size() * sizeof(double)));
+  HIP_TRY(hipMemcpy(c->d_elem, el.data(), el.size() * sizeof(double), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -136,13 +163,10 @@ int check_grid(const tgr_grid* g, int nsp) {
   return 0;
 }
 
-int run_step(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const tgr_grid* g, double* rho,
-             double* sc, int isc_sc0, int isc_T0, const int* mask, double* hr, double* chem, hipStream_t s,
-             tgr_stats* stats) {
-  if (!(dt > 0.0) || !std::isfinite(dt)) return fail(TGR_E_ARG, "dt must be positive and finite");
+// step parameters of a block (no queue / workspace yet)
+StepParams step_params(const tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const tgr_grid* g,
+                       double* rho, double* sc, int isc_sc0, int isc_T0, const int* mask, double* hr, double* chem) {
   const int nx = g->imax_ - g->imin_ + 1, ny = g->jmax_ - g->jmin_ + 1, nz = g->kmax_ - g->kmin_ + 1;
-  if (stats) memset(stats, 0, sizeof(*stats));
-  if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
   const int64_t nxo = g->imaxo_ - g->imino_ + 1, nyo = g->jmaxo_ - g->jmino_ + 1, nzo = g->kmaxo_ - g->kmino_ + 1;
   StepParams p{};
   p.m = c->dm;
@@ -176,15 +200,45 @@ int run_step(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const
   p.ncell = (int64_t)nx * ny * nz;
   p.totals = c->d_totals;
   p.cell_stats = nullptr;
-  int per_cu = 1;
-  if (c->grid_override <= 0) {
-    const int nb = tgr_step_occupancy(c->neq);
-    if (nb > 0) per_cu = nb;
+  return p;
+}
+
+// active (mask /= 1) interior cells of a block
+int count_cells(const tgr_grid* g, const int* mask, int64_t mask_ld, int64_t ncell, int64_t* cells) {
+  if (!mask) {
+    *cells = ncell;
+    return 0;
   }
-  // persistent grid: every wave of every resident workgroup pulls cells
+  std::vector<int> hm((size_t)(g->imaxo - g->imino + 1) * (g->jmaxo - g->jmino + 1));
+  HIP_TRY(hipMemcpy(hm.data(), mask, hm.size() * sizeof(int), hipMemcpyDefault));
+  int64_t n = 0;
+  for (int j = g->jmin_; j <= g->jmax_; ++j)
+    for (int i = g->imin_; i <= g->imax_; ++i) n += hm[(size_t)(j - g->jmino) * mask_ld + (i - g->imino)] != 1;
+  *cells = n * (g->kmax_ - g->kmin_ + 1);
+  return 0;
+}
+
+// persistent grid of a solver kernel: every wave of every resident workgroup pulls cells
+int persistent_grid(const tgr_chem* c, int occupancy, int64_t ncell) {
+  const int per_cu = (c->grid_override <= 0 && occupancy > 0) ? occupancy : 1;
   const int wpb = tgr_step_block_waves();
   const int64_t gs = c->grid_override > 0 ? c->grid_override : (int64_t)c->n_cu * per_cu;
-  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(gs, (p.ncell + wpb - 1) / wpb));
+  return (int)std::max<int64_t>(1, std::min<int64_t>(gs, (ncell + wpb - 1) / wpb));
+}
+
+// the plain step; with list / nlist only the listed cells (device list and count)
+int run_step(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const tgr_grid* g, double* rho,
+             double* sc, int isc_sc0, int isc_T0, const int* mask, double* hr, double* chem, hipStream_t s,
+             tgr_stats* stats, const int* list = nullptr, const unsigned int* nlist = nullptr) {
+  if (!(dt > 0.0) || !std::isfinite(dt)) return fail(TGR_E_ARG, "dt must be positive and finite");
+  const int nx = g->imax_ - g->imin_ + 1, ny = g->jmax_ - g->jmin_ + 1, nz = g->kmax_ - g->kmin_ + 1;
+  if (stats) memset(stats, 0, sizeof(*stats));
+  if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
+  StepParams p = step_params(c, dt, p_thermo, p_thermo_old, g, rho, sc, isc_sc0, isc_T0, mask, hr, chem);
+  p.cell_list = list;
+  p.nlist = nlist;
+  const int grid = persistent_grid(c, tgr_step_occupancy(c->neq), p.ncell);
+  const int wpb = tgr_step_block_waves();
   int rc0 = c->b_ws.ensure((size_t)grid * wpb * c->neq * 64 * sizeof(double));
   if (rc0) return rc0;
   p.queue = c->d_queue;
@@ -202,18 +256,8 @@ int run_step(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     int64_t cells = 0;
-    if (mask) {
-      // active cells: those the kernel integrated (counted through nfe > 0 is
-      // not robust for failures), recomputed on the host from the mask
-      std::vector<int> hm((size_t)(g->imaxo - g->imino + 1) * (g->jmaxo - g->jmino + 1));
-      HIP_TRY(hipMemcpy(hm.data(), mask, hm.size() * sizeof(int), hipMemcpyDefault));
-      for (int j = g->jmin_; j <= g->jmax_; ++j)
-        for (int i = g->imin_; i <= g->imax_; ++i)
-          cells += hm[(size_t)(j - g->jmino) * p.mask_ld + (i - g->imino)] != 1;
-      cells *= nz;
-    } else {
-      cells = p.ncell;
-    }
+    // active cells, recomputed on the host from the mask
+    if ((rc = count_cells(g, mask, p.mask_ld, p.ncell, &cells))) return rc;
     stats->cells = cells;
     stats->nst = tot[0];
     stats->nfe = tot[1];
@@ -275,10 +319,18 @@ int tgr_chem_destroy(tgr_chem* c) {
   if (!c) return 0;
   if (c->d_tables) (void)hipFree(c->d_tables);
   if (c->d_props) (void)hipFree(c->d_props);
+  if (c->d_elem) (void)hipFree(c->d_elem);
   if (c->d_queue) (void)hipFree(c->d_queue);
   if (c->d_totals) (void)hipFree(c->d_totals);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->ev2) (void)hipEventDestroy(c->ev2);
+  for (void* q : {(void*)c->d_in_shift, (void*)c->d_in_scale, (void*)c->d_out_scale, (void*)c->d_rej_count,
+                  (void*)c->d_stotals})
+    if (q) (void)hipFree(q);
+  c->b_x.release();
+  c->b_guess.release();
+  c->b_rej.release();
   c->b_ws.release();
   c->b_rho.release();
   c->b_sc.release();
@@ -319,6 +371,20 @@ int tgr_chem_set_option(tgr_chem* c, const char* key, int64_t v) {
   } else if (!strcmp(key, "max_steps")) {
     if (v < 1) return fail(TGR_E_ARG, "max_steps must be positive");
     c->max_steps = (int)v;
+  } else if (!strcmp(key, "surr_nsub")) {
+    if (v < 1 || v > tgr::kSurrMaxSub) return fail(TGR_E_ARG, "surr_nsub must be 1..8");
+    c->surr_nsub = (int)v;
+  } else if (!strcmp(key, "surr_guess")) {
+    if (v != 0 && v != 1) return fail(TGR_E_ARG, "surr_guess must be 0 or 1");
+    c->surr_guess = (int)v;
+  } else if (!strcmp(key, "surr_probe")) {
+    if (v < 0 || v > 1000000) return fail(TGR_E_ARG, "surr_probe must be >= 0");
+    c->surr_probe = (int)v;
+    c->surr_skipped = 0;
+    c->surr_last_accept = 1.0;
+  } else if (!strcmp(key, "surr_newton_max")) {
+    if (v < 1 || v > 50) return fail(TGR_E_ARG, "surr_newton_max must be 1..50");
+    c->surr_newton_max = (int)v;
   } else {
     return fail(TGR_E_ARG, std::string("unknown option: ") + key);
   }
@@ -333,6 +399,224 @@ int tgr_chem_source_chemistry_device(tgr_chem* c, double dt, double p_thermo, do
   if (rc) return rc;
   return run_step(c, dt, p_thermo, p_thermo_old, g, RHO_, SC_, g->isc_sc - 1, g->isc_T - 1, mask, HR_gp,
                   chemsrc_gp, (hipStream_t)stream, stats);
+}
+
+int tgr_chem_set_surrogate(tgr_chem* c, tgr_mlp* net, const float* in_shift, const float* in_scale,
+                           const double* out_scale) {
+  if (!c) return fail(TGR_E_ARG, "NULL context");
+  c->net = nullptr;
+  if (!net) return 0;
+  if (!in_shift || !in_scale || !out_scale) return fail(TGR_E_ARG, "NULL scaling array");
+  int nin = 0, nout = 0;
+  if (tgr_mlp_shape(net, &nin, &nout, nullptr, nullptr)) return fail(TGR_E_ARG, "bad network handle");
+  if (nin != c->nsp + 2 || nout != c->neq)
+    return fail(TGR_E_ARG, "network shape must be nsp + 2 inputs (log10 Y, T, log10 dt) and nsp + 1 outputs");
+  int64_t np = 0;
+  if (tgr_mlp_layout(net, &np, &c->net_in_pad, &c->net_out_pad)) return fail(TGR_E_ARG, "bad network handle");
+  for (int k = 0; k < nin; ++k)
+    if (!std::isfinite(in_shift[k]) || !std::isfinite(in_scale[k])) return fail(TGR_E_ARG, "non-finite input scaling");
+  for (int k = 0; k < nout; ++k)
+    if (!std::isfinite(out_scale[k])) return fail(TGR_E_ARG, "non-finite output scaling");
+  c->h_in_shift.assign(c->net_in_pad, 0.0f);
+  c->h_in_scale.assign(c->net_in_pad, 0.0f);
+  for (int k = 0; k < nin; ++k) {
+    c->h_in_shift[k] = in_shift[k];
+    c->h_in_scale[k] = in_scale[k];
+  }
+  if (!c->d_in_shift) HIP_TRY(hipMalloc(&c->d_in_shift, 64 * 4 * 4));
+  if (!c->d_in_scale) HIP_TRY(hipMalloc(&c->d_in_scale, 64 * 4 * 4));
+  if (!c->d_out_scale) HIP_TRY(hipMalloc(&c->d_out_scale, 64 * 8));
+  if (!c->d_rej_count) HIP_TRY(hipMalloc((void**)&c->d_rej_count, 256));
+  if (!c->d_stotals) HIP_TRY(hipMalloc((void**)&c->d_stotals, 256));
+  if (!c->ev2) HIP_TRY(hipEventCreate(&c->ev2));
+  if (c->net_in_pad > 256) return fail(TGR_E_ARG, "network input too wide");
+  HIP_TRY(hipMemcpy(c->d_in_shift, c->h_in_shift.data(), c->net_in_pad * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_in_scale, c->h_in_scale.data(), c->net_in_pad * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_out_scale, out_scale, c->neq * 8, hipMemcpyHostToDevice));
+  c->net = net;
+  c->surr_last_accept = 1.0;
+  c->surr_skipped = 0;
+  return 0;
+}
+
+int tgr_chem_set_option_real(tgr_chem* c, const char* key, double v) {
+  if (!c || !key) return fail(TGR_E_ARG, "NULL argument");
+  if (!(v > 0.0) || !std::isfinite(v)) return fail(TGR_E_ARG, "value must be positive and finite");
+  if (!strcmp(key, "surr_tol_y")) c->surr_tol_y = v;
+  else if (!strcmp(key, "surr_tol_t")) c->surr_tol_t = v;
+  else if (!strcmp(key, "surr_newton_tol")) c->surr_newton_tol = v;
+  else if (!strcmp(key, "surr_min_accept")) {
+    if (v > 1.0) return fail(TGR_E_ARG, "surr_min_accept must be in (0, 1]");
+    c->surr_min_accept = v;
+    c->surr_skipped = 0;
+    c->surr_last_accept = 1.0;
+  } else if (!strcmp(key, "surr_grade")) {
+    if (v < 1.0 || v > 16.0) return fail(TGR_E_ARG, "surr_grade must be in [1, 16]");
+    c->surr_grade = v;
+  }
+  else return fail(TGR_E_ARG, std::string("unknown option: ") + key);
+  return 0;
+}
+
+int tgr_chem_surrogate_features_device(tgr_chem* c, double dt, const tgr_grid* g, const double* SC_, uint16_t* X,
+                                       int64_t rows, void* stream) {
+  if (!c || !SC_ || !X) return fail(TGR_E_ARG, "NULL argument");
+  if (!c->net) return fail(TGR_E_ARG, "no network set (tgr_chem_set_surrogate)");
+  int rc = check_grid(g, c->nsp);
+  if (rc) return rc;
+  if (!(dt > 0.0) || !std::isfinite(dt)) return fail(TGR_E_ARG, "dt must be positive and finite");
+  StepParams p = step_params(c, dt, 0.0, 0.0, g, nullptr, (double*)SC_, g->isc_sc - 1, g->isc_T - 1, nullptr, nullptr,
+                             nullptr);
+  if (p.ncell <= 0) return 0;
+  if (rows < p.ncell || rows % 64) return fail(TGR_E_ARG, "rows must be >= the interior cells and a multiple of 64");
+  if (tgr_launch_surr_features(&p, c->neq, rows, c->net_in_pad, c->d_in_shift, c->d_in_scale, (float)std::log10(dt),
+                               X, (hipStream_t)stream))
+    return fail(TGR_E_HIP, "feature kernel launch failed");
+  return 0;
+}
+
+int tgr_chem_source_chemistry_surrogate_device(tgr_chem* c, double dt, double p_thermo, double p_thermo_old,
+                                               const tgr_grid* g, double* RHO_, double* SC_, const int* mask,
+                                               double* HR_gp, double* chemsrc_gp, void* stream, tgr_stats* stats,
+                                               tgr_surrogate_stats* sstats) {
+  if (!c || !RHO_ || !SC_) return fail(TGR_E_ARG, "NULL argument");
+  if (!c->net) return fail(TGR_E_ARG, "no network set (tgr_chem_set_surrogate)");
+  int rc = check_grid(g, c->nsp);
+  if (rc) return rc;
+  if (!(dt > 0.0) || !std::isfinite(dt)) return fail(TGR_E_ARG, "dt must be positive and finite");
+  if (stats) memset(stats, 0, sizeof(*stats));
+  if (sstats) memset(sstats, 0, sizeof(*sstats));
+  hipStream_t s = (hipStream_t)stream;
+  StepParams p = step_params(c, dt, p_thermo, p_thermo_old, g, RHO_, SC_, g->isc_sc - 1, g->isc_T - 1, mask, HR_gp,
+                             chemsrc_gp);
+  if (p.nx <= 0 || p.ny <= 0 || p.nz <= 0) return 0;
+  if (c->surr_probe > 0 && c->surr_last_accept < c->surr_min_accept && c->surr_skipped + 1 < c->surr_probe) {
+    // the guided solve has not been paying: plain step only
+    c->surr_skipped++;
+    if (sstats) sstats->guided = 0;
+    rc = run_step(c, dt, p_thermo, p_thermo_old, g, RHO_, SC_, g->isc_sc - 1, g->isc_T - 1, mask, HR_gp, chemsrc_gp,
+                  s, stats);
+    if (!rc && sstats && stats) {
+      sstats->rejected = stats->cells;
+      sstats->fallback_ms = stats->kernel_ms;
+    }
+    return rc;
+  }
+  c->surr_skipped = 0;
+  if (sstats) sstats->guided = 1;
+  const int64_t rows = (p.ncell + 63) / 64 * 64;
+  const int nst = tgr::kSurrStages * c->surr_nsub;
+  // sub-step sizes: geometric with ratio surr_grade (1: equal), summing to dt
+  double hsub[tgr::kSurrMaxSub] = {0.0};
+  {
+    const int n = c->surr_nsub;
+    const double r = c->surr_grade;
+    double acc = 0.0;
+    for (int i = 0; i < n; ++i) {
+      hsub[i] = (r == 1.0) ? dt / n : dt * std::pow(r, (double)i) * (r - 1.0) / (std::pow(r, (double)n) - 1.0);
+      acc += hsub[i];
+    }
+    hsub[n - 1] += dt - acc;  // exact sum
+  }
+  const int wpb = tgr_step_block_waves();
+  const int grid_s = persistent_grid(c, tgr_surr_occupancy(c->neq), p.ncell);
+  const int grid_p = persistent_grid(c, tgr_step_occupancy(c->neq), p.ncell);
+  // every buffer sized before the first launch (a resize frees memory)
+  if ((rc = c->b_ws.ensure((size_t)std::max(grid_s, grid_p) * wpb * c->neq * 64 * sizeof(double))) ||
+      (rc = c->b_x.ensure((size_t)rows * c->net_in_pad * 2)) ||
+      (rc = c->b_guess.ensure((size_t)nst * rows * c->net_out_pad * 4)) ||
+      (rc = c->b_rej.ensure((size_t)p.ncell * sizeof(int))))
+    return rc;
+  const bool timing = stats || sstats || c->surr_probe > 0;
+  if (timing) HIP_TRY(hipEventRecord(c->ev0, s));
+  uint16_t* X = (uint16_t*)c->b_x.p;
+  float* guess = (float*)c->b_guess.p;
+  if (c->surr_guess) {
+    if (tgr_launch_surr_features(&p, c->neq, rows, c->net_in_pad, c->d_in_shift, c->d_in_scale,
+                                 (float)std::log10(dt), X, s))
+      return fail(TGR_E_HIP, "feature kernel launch failed");
+    const int col = c->nsp + 1;
+    static const double cst[tgr::kSurrStages] = {0.43586652150845899942, 0.5 * (1.0 + 0.43586652150845899942), 1.0};
+    double t0 = 0.0;
+    for (int k = 0; k < nst; ++k) {
+      const int i = k / tgr::kSurrStages, j = k % tgr::kSurrStages;
+      if (j == 0 && i > 0) t0 += hsub[i - 1];
+      const double tau = t0 + cst[j] * hsub[i];
+      const float v = (float)((std::log10(tau) - c->h_in_shift[col]) * c->h_in_scale[col]);
+      if (tgr_launch_surr_set_column(X, rows, c->net_in_pad, col, v, s)) return fail(TGR_E_HIP, "column launch");
+      if (tgr_mlp_forward(c->net, X, rows, guess + (size_t)k * rows * c->net_out_pad, s))
+        return fail(TGR_E_HIP, std::string("network forward failed: ") + tgr_mlp_last_error());
+    }
+  }
+  SurrParams P{};
+  P.s = p;
+  P.s.queue = c->d_queue;
+  P.s.ws = (double*)c->b_ws.p;
+  P.guess = guess;
+  P.guess_stride = rows * c->net_out_pad;
+  P.guess_ld = c->net_out_pad;
+  for (int i = 0; i < tgr::kSurrMaxSub; ++i) P.hsub[i] = hsub[i];
+  P.out_scale = c->d_out_scale;
+  P.elem_w = c->d_elem;
+  P.nsub = c->surr_nsub;
+  P.use_guess = c->surr_guess;
+  P.newton_max = c->surr_newton_max;
+  P.newton_tol = c->surr_newton_tol;
+  P.tol_y = c->surr_tol_y;
+  P.tol_t = c->surr_tol_t;
+  P.rej_count = c->d_rej_count;
+  P.rej_list = (int*)c->b_rej.p;
+  P.stotals = c->d_stotals;
+  if (tgr_launch_surr(&P, c->neq, grid_s, s)) return fail(TGR_E_HIP, "surrogate kernel launch failed");
+  if (timing) HIP_TRY(hipEventRecord(c->ev2, s));
+  // the plain integrator on the rejected cells
+  if ((rc = run_step(c, dt, p_thermo, p_thermo_old, g, RHO_, SC_, g->isc_sc - 1, g->isc_T - 1, mask, HR_gp,
+                     chemsrc_gp, s, nullptr, P.rej_list, c->d_rej_count)))
+    return rc;
+  // the accepted fraction steers the adaptive use: read back after every
+  // guided call (a small copy; the call is then synchronous)
+  const bool need_host = stats || sstats || c->surr_probe > 0;
+  if (need_host) {
+    HIP_TRY(hipEventRecord(c->ev1, s));
+    unsigned long long tot[TGR_NCOUNT], st[tgr::kSurrNCount];
+    HIP_TRY(hipMemcpyAsync(tot, c->d_totals, sizeof(tot), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(st, c->d_stotals, sizeof(st), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    float ms_s = 0.f, ms_p = 0.f;
+    if (stats || sstats) {
+      HIP_TRY(hipEventElapsedTime(&ms_s, c->ev0, c->ev2));
+      HIP_TRY(hipEventElapsedTime(&ms_p, c->ev2, c->ev1));
+    }
+    c->surr_last_accept = (double)st[0] / (double)std::max<unsigned long long>(1ull, st[0] + st[1]);
+    if (stats) {
+      int64_t cells = 0;
+      if ((rc = count_cells(g, mask, p.mask_ld, p.ncell, &cells))) return rc;
+      stats->cells = cells;
+      stats->nst = tot[0];
+      stats->nfe = tot[1];
+      stats->nfe_dq = tot[2];
+      stats->nje = tot[3];
+      stats->nsetups = tot[4];
+      stats->nni = tot[5];
+      stats->netf = tot[6];
+      stats->ncfn = tot[7];
+      stats->nlu = tot[8];
+      stats->failed = tot[9];
+      stats->kernel_ms = ms_s + ms_p;
+    }
+    if (sstats) {
+      sstats->accepted = st[0];
+      sstats->rejected = st[1];
+      sstats->newton_iters = st[2];
+      sstats->rhs_evals = st[3];
+      sstats->rej_newton = st[4];
+      sstats->rej_error = st[5];
+      sstats->rej_enthalpy = st[6];
+      sstats->surrogate_ms = ms_s;
+      sstats->fallback_ms = ms_p;
+    }
+  }
+  return 0;
 }
 
 int tgr_chem_source_chemistry(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const tgr_grid* g,

@@ -38,11 +38,39 @@ struct StepParams {
   double* chemsrc;  // chemsrc_gp(..., 3) or null
   int idx_h2o, idx_co, idx_co2;
   int64_t ncell;
+  const int* cell_list;          // cells to integrate (indices into [0, ncell)) or null: all
+  const unsigned int* nlist;     // with cell_list: device count of listed cells
   unsigned int* queue;           // device work counter
   double* ws;                    // per-block workspace: saved Jacobian, neq * 64 doubles per block
   int* cell_stats;               // [ncell][TGR_NCOUNT] or null
   unsigned long long* totals;    // [TGR_NCOUNT] (last: failed cells) or null
 };
+
+// Surrogate-guided step: every interior cell gets a one-step SDIRK solve
+// whose Newton iterations start from the network's predictions; cells that
+// fail its checks are listed for the plain integrator.
+constexpr int kSurrMaxSub = 8;   // sub-steps of the guided solve
+constexpr int kSurrStages = 3;   // stages per sub-step
+struct SurrParams {
+  StepParams s;                  // field addressing, tolerances, queue, workspace
+  const float* guess;            // network outputs, stage j of sub-step i at guess + (i*3 + j) * guess_stride
+  int64_t guess_stride;          // rows * guess_ld
+  int64_t guess_ld;
+  const double* out_scale;       // [neq]: increment of Y_k / T per unit network output
+  const double* elem_w;          // [TGR_NELEM][64]: atoms of element e per molecule of species k / W_k
+  int nsub;                      // sub-steps (1..kSurrMaxSub)
+  double hsub[kSurrMaxSub];      // their sizes (sum dt)
+  int use_guess;                 // 0: Newton starts from the previous stage value
+  int newton_max;
+  double newton_tol;             // Newton stop: |correction| <= newton_tol x acceptance tolerance
+  double tol_y, tol_t;           // acceptance: |error of Y_k| <= tol_y, |error of T| <= tol_t K
+  unsigned int* rej_count;       // cells handed to the plain integrator
+  int* rej_list;
+  unsigned long long* stotals;   // [kSurrNCount]
+};
+// surrogate counters: accepted, rejected, Newton iterations, RHS evaluations,
+// rejected for Newton failure, for the error estimate, for the enthalpy check
+constexpr int kSurrNCount = 8;
 
 // Transport / equation-of-state properties of a whole Fortran-layout block.
 // ptab: mucoeff[64], kOverEps[64], W[64], then A[nsp][64] and B[nsp][64]

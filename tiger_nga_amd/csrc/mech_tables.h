@@ -55,7 +55,10 @@ typedef struct {
   const tgr_nu *nu;     /* [n_nu] */
   const double *mucoeff; /* [nsp] GETMUCOEFF, kg/(m s) */
   const double *koveps;  /* [nsp] GETKOVEREPS, 1/K */
+  const double *elem;    /* [TGR_NELEM][nsp] atoms of C, H, O, N, AR per molecule */
 } tgr_mech_tables;
+
+#define TGR_NELEM 5
 
 #ifdef __cplusplus
 }

@@ -324,8 +324,9 @@ GemmArgs plain_args() {
   return a;
 }
 
-// forward over B rows of x (bf16 [B][fin[0]]); keeps what the backward needs
-int forward(tgr_mlp* h, const uint16_t* x, int64_t B, float* out, int64_t ldo, hipStream_t s) {
+// forward over B rows of x (bf16 [B][fin[0]]); keep_pre: store the hidden
+// pre-activations that the backward needs
+int forward(tgr_mlp* h, const uint16_t* x, int64_t B, float* out, int64_t ldo, bool keep_pre, hipStream_t s) {
   const uint16_t* in = x;
   for (int l = 0; l < h->nlin; ++l) {
     GemmArgs a = plain_args();
@@ -342,7 +343,7 @@ int forward(tgr_mlp* h, const uint16_t* x, int64_t B, float* out, int64_t ldo, h
     a.n_valid = h->fout[l];
     if (l < h->nlin - 1) {
       a.act = kGelu;
-      a.pre = h->pre[l];
+      a.pre = keep_pre ? h->pre[l] : nullptr;
       a.ldpre = h->hidden;
       a.out_h = h->act[l + 1];
       a.ldh = h->fin[l + 1];
@@ -363,7 +364,7 @@ int loss_and_grad(tgr_mlp* h, const uint16_t* x, const float* tgt, int64_t B, hi
   int rc = h->ensure(B);
   if (rc) return rc;
   const int ldo = h->fout.back();
-  if ((rc = forward(h, x, B, h->out, ldo, s))) return rc;
+  if ((rc = forward(h, x, B, h->out, ldo, true, s))) return rc;
   hipLaunchKernelGGL(mse_grad_kernel, dim3(tgr_mlp::kLossBlocks), dim3(256), 0, s, h->out, (int64_t)ldo, tgt,
                      h->nout, B, h->dyf[0], h->dyh[0], h->part);
   hipLaunchKernelGGL(finish_loss_kernel, dim3(1), dim3(256), 0, s, h->part, tgr_mlp::kLossBlocks,
@@ -416,6 +417,8 @@ int loss_and_grad(tgr_mlp* h, const uint16_t* x, const float* tgt, int64_t B, hi
   }
   return 0;
 }
+
+constexpr int64_t kInferChunk = 262144;  // rows per inference pass (a multiple of 64)
 
 int check_batch(const tgr_mlp* h, int64_t B) {
   if (!h) return fail(TGR_MLP_E_ARG, "null network");
@@ -482,6 +485,15 @@ int tgr_mlp_layout(const tgr_mlp* h, int64_t* nparam, int* in_pad, int* out_pad)
   return 0;
 }
 
+int tgr_mlp_shape(const tgr_mlp* h, int* nin, int* nout, int* hidden, int* n_hidden_layers) {
+  if (!h) return fail(TGR_MLP_E_ARG, "null network");
+  if (nin) *nin = h->nin;
+  if (nout) *nout = h->nout;
+  if (hidden) *hidden = h->hidden;
+  if (n_hidden_layers) *n_hidden_layers = h->nhl;
+  return 0;
+}
+
 int tgr_mlp_set_params(tgr_mlp* h, const float* host) {
   if (!h || !host) return fail(TGR_MLP_E_ARG, "null argument");
   // padded entries must be zero for the padded network to equal the unpadded one
@@ -516,8 +528,15 @@ int tgr_mlp_forward(tgr_mlp* h, const uint16_t* x, int64_t B, float* out, void* 
   int rc = check_batch(h, B);
   if (rc) return rc;
   if (!x || !out) return fail(TGR_MLP_E_ARG, "null argument");
-  if ((rc = h->ensure(B))) return rc;
-  return forward(h, x, B, out, h->fout.back(), (hipStream_t)stream);
+  // inference in row chunks: the workspace stays at most kInferChunk rows
+  const int64_t chunk = std::min<int64_t>(B, kInferChunk);
+  if ((rc = h->ensure(chunk))) return rc;
+  const int ldo = h->fout.back();
+  for (int64_t r0 = 0; r0 < B; r0 += chunk) {
+    const int64_t rows = std::min<int64_t>(chunk, B - r0);
+    if ((rc = forward(h, x + r0 * h->fin[0], rows, out + r0 * ldo, ldo, false, (hipStream_t)stream))) return rc;
+  }
+  return 0;
 }
 
 int tgr_mlp_gradients(tgr_mlp* h, const uint16_t* x, const float* tgt, int64_t B, float* grad_host,

@@ -39,6 +39,12 @@ class _Stats(ctypes.Structure):
         ("kernel_ms", ctypes.c_double)]
 
 
+class _SurrStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in (
+        "accepted", "rejected", "newton_iters", "rhs_evals", "rej_newton", "rej_error", "rej_enthalpy")] + [
+        ("surrogate_ms", ctypes.c_double), ("fallback_ms", ctypes.c_double), ("guided", ctypes.c_int)]
+
+
 class _Info(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
         "nsp", "nrxn", "nslot", "n_third", "n_falloff", "n_third_body_rxn", "n_reversible", "n_nu", "n_fc_terms",
@@ -133,6 +139,13 @@ def load_library(path: str = LIB_PATH):
     lib.tgr_chem_properties_device.argtypes = [vp, ctypes.c_double, ctypes.POINTER(_Grid), vp, vp, vp, ctypes.c_int,
                                                vp, vp, vp, vp]
     lib.tgr_chem_set_option.argtypes = [vp, ctypes.c_char_p, ctypes.c_int64]
+    lib.tgr_chem_set_option_real.argtypes = [vp, ctypes.c_char_p, ctypes.c_double]
+    fp = ctypes.POINTER(ctypes.c_float)
+    lib.tgr_chem_set_surrogate.argtypes = [vp, vp, fp, fp, dp]
+    lib.tgr_chem_surrogate_features_device.argtypes = [vp, ctypes.c_double, ctypes.POINTER(_Grid), vp, vp,
+                                                       ctypes.c_int64, vp]
+    lib.tgr_chem_source_chemistry_surrogate_device.argtypes = step_args + [vp, ctypes.POINTER(_Stats),
+                                                                           ctypes.POINTER(_SurrStats)]
     lib.tgr_chem_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -198,8 +211,54 @@ class FiniteChem:
         except Exception:
             pass
 
-    def set_option(self, key: str, value: int):
-        _check(self.lib, self.lib.tgr_chem_set_option(self._h, key.encode(), int(value)), "set_option")
+    def set_option(self, key: str, value):
+        """Integer knobs ("grid", "rhs_mapping", "max_order", "max_steps",
+        "surr_nsub", "surr_guess", "surr_newton_max", "surr_probe") or real ones
+        ("surr_tol_y", "surr_tol_t", "surr_newton_tol", "surr_grade", "surr_min_accept")."""
+        if key in ("surr_tol_y", "surr_tol_t", "surr_newton_tol", "surr_grade", "surr_min_accept"):
+            _check(self.lib, self.lib.tgr_chem_set_option_real(self._h, key.encode(), float(value)), "set_option")
+        else:
+            _check(self.lib, self.lib.tgr_chem_set_option(self._h, key.encode(), int(value)), "set_option")
+
+    def set_surrogate(self, net, in_shift, in_scale, out_scale):
+        """Attach a trained tiger_nga_amd.mlp.MLP (None detaches) with its
+        input shift/scale (nsp + 2) and output scale (nsp + 1)."""
+        self._net = net
+        if net is None:
+            _check(self.lib, self.lib.tgr_chem_set_surrogate(self._h, None, None, None, None), "set_surrogate")
+            return
+        fs = np.ascontiguousarray(in_shift, dtype=np.float32)
+        fc = np.ascontiguousarray(in_scale, dtype=np.float32)
+        oc = np.ascontiguousarray(out_scale, dtype=np.float64)
+        fptr = ctypes.POINTER(ctypes.c_float)
+        _check(self.lib, self.lib.tgr_chem_set_surrogate(
+            self._h, net._h, fs.ctypes.data_as(fptr), fc.ctypes.data_as(fptr),
+            oc.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "set_surrogate")
+
+    def surrogate_features(self, SC_, grid: Grid, dt: float, X, stream=None):
+        """Network inputs of the block's interior cells into X (bfloat16
+        [rows, in_pad], rows a multiple of 64 >= cells)."""
+        g = grid._c()
+        _check(self.lib, self.lib.tgr_chem_surrogate_features_device(self._h, dt, ctypes.byref(g), _ptr(SC_),
+                                                                     _ptr(X), X.shape[0], stream), "features")
+
+    def source_chemistry_surrogate(self, RHO_, SC_, grid: Grid, dt: float, p_thermo: float,
+                                   p_thermo_old: float | None = None, mask=None, HR_gp=None, chemsrc_gp=None,
+                                   stream=None, sync: bool = True):
+        """The surrogate-guided step on device tensors; returns (step stats,
+        guided-solve stats) when sync, else None."""
+        if p_thermo_old is None:
+            p_thermo_old = p_thermo
+        self._check_arrays(RHO_, SC_, grid, mask, HR_gp, chemsrc_gp, True)
+        st, ss = _Stats(), _SurrStats()
+        g = grid._c()
+        rc = self.lib.tgr_chem_source_chemistry_surrogate_device(
+            self._h, dt, p_thermo, p_thermo_old, ctypes.byref(g), _ptr(RHO_), _ptr(SC_), _ptr(mask), _ptr(HR_gp),
+            _ptr(chemsrc_gp), stream, ctypes.byref(st) if sync else None, ctypes.byref(ss) if sync else None)
+        _check(self.lib, rc, "source_chemistry_surrogate")
+        if not sync:
+            return None
+        return ({n: getattr(st, n) for n, _ in _Stats._fields_}, {n: getattr(ss, n) for n, _ in _SurrStats._fields_})
 
     def source_chemistry(self, RHO_, SC_, grid: Grid, dt: float, p_thermo: float, p_thermo_old: float | None = None,
                          mask=None, HR_gp=None, chemsrc_gp=None, stream=None, sync: bool = True):

@@ -776,6 +776,12 @@ def emit_c_header(mech: dict) -> str:
     w("static const double %s_W[%d] = {%s};" % (n, nsp, ", ".join(_c(x) for x in mech["W"])))
     w("static const double %s_mucoeff[%d] = {%s};" % (n, nsp, ", ".join(_c(x) for x in mech["mucoeff"])))
     w("static const double %s_koveps[%d] = {%s};" % (n, nsp, ", ".join(_c(x) for x in mech["koveps"])))
+    # element composition (C, H, O, N, AR) from the species names
+    from .meta import ELEMENTS, composition
+    erows = []
+    for el in ELEMENTS:
+        erows.append("{" + ", ".join("%d" % composition(sp).get(el, 0) for sp in mech["species"]) + "}")
+    w("static const double %s_elem[%d][%d] = {\n  %s};" % (n, len(ELEMENTS), nsp, ",\n  ".join(erows)))
     # thermo: [range][species][13]: a_h, c1..c5, c6, a_cp, d1..d5
     rows = []
     for blk in ("thermo_hi", "thermo_lo"):
@@ -825,8 +831,8 @@ def emit_c_header(mech: dict) -> str:
     w("")
     w("static const tgr_mech_tables %s_tables = {" % n)
     w('  "%s", %d, %d, %d, %d, %d, %s,' % (n, nsp, len(rx), mech["nslot"], nm, len(nus), _c(mech["tmid"])))
-    w("  %s_species, %s_W, &%s_thermo[0][0], &%s_eff[0][0], %s_rxn, %s_nu, %s_mucoeff, %s_koveps};"
-      % (n, n, n, n, n, n, n, n))
+    w("  %s_species, %s_W, &%s_thermo[0][0], &%s_eff[0][0], %s_rxn, %s_nu, %s_mucoeff, %s_koveps, &%s_elem[0][0]};"
+      % (n, n, n, n, n, n, n, n, n))
     return "\n".join(L) + "\n"
 
 
