@@ -77,13 +77,14 @@ def test_forward_matches_reference(torch_dev):
           % (err, np.abs(got - exact).max() / np.abs(exact).max()))
 
 
-def test_gradients_match_reference(torch_dev):
+@pytest.mark.parametrize("B", [128, 1024])
+def test_gradients_match_reference(torch_dev, B):
     """Every parameter's gradient: relative L2 error per tensor against the
     FP64 reference with the same bf16 rounding points (tight), and against
     the exact FP64 gradient of the same parameters (loose: bf16 operands)."""
     torch = torch_dev
     from tiger_nga_amd import mlp
-    net, flat, params, x, tgt = _net_case(torch)
+    net, flat, params, x, tgt = _net_case(torch, B=B)   # B = 1024: weight gradients split over the batch
     g, loss = net.gradients(_bf(torch, x), torch.from_numpy(tgt).cuda())
     lq, gq = mlp_ref.loss_and_grad(params, x, tgt, 37, quantize=True)
     le, ge = mlp_ref.loss_and_grad(params, x, tgt, 37, quantize=False)

@@ -83,6 +83,7 @@ struct GemmArgs {
   uint16_t* out_h;  // bf16 out (or null)
   int64_t ldh;
   int n_valid;  // columns n >= n_valid are written as 0
+  int ksplit;   // > 1: workgroup z sums K range z of ksplit into out_f + z * M * ldf (no epilogue)
 };
 
 // AK: A's k stride is 1 (else its m stride is 1); BK: B's k stride is 1 (else its n stride is 1).
@@ -94,11 +95,13 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
   const int lane = t & 63, w = t >> 6;
   const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  const int kchunk = g.K / g.ksplit;
+  const int kbeg = blockIdx.z * kchunk, kend = kbeg + kchunk;
   f32x4 acc[2][2];
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  for (int k0 = 0; k0 < g.K; k0 += kKStep) {
+  for (int k0 = kbeg; k0 < kend; k0 += kKStep) {
     // global -> registers: 8 bf16 of A and 8 of B per thread (16-byte loads)
     uint4 ra, rb;
     int ar, ac, br, bc;
@@ -147,6 +150,15 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
   }
 
   // epilogue: lane l holds rows (l/16)*4 + 0..3, column l%16 of each 16x16 block
+  if (g.ksplit > 1) {  // partial sums of this K range
+    float* part = g.out_f + (int64_t)blockIdx.z * g.M * g.ldf;
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) {
+        const int col = n0 + wn + j * 16 + (lane & 15);
+        for (int q = 0; q < 4; ++q) part[(int64_t)(m0 + wm + i * 16 + (lane >> 4) * 4 + q) * g.ldf + col] = acc[i][j][q];
+      }
+    return;
+  }
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) {
       const int col = n0 + wn + j * 16 + (lane & 15);
@@ -208,15 +220,28 @@ __global__ __launch_bounds__(256) void finish_loss_kernel(const float* part, int
   if (threadIdx.x == 0) loss[0] = red[0] * inv_count;
 }
 
-// out[n] = sum over rows of y[row][n], 64 columns per workgroup, 4 row groups
-__global__ __launch_bounds__(256) void colsum_kernel(const float* y, int64_t ld, int64_t rows, float* out) {
+// part[chunk][n] = sum over the rows of chunk `blockIdx.y` of y[row][n]; 64
+// columns per workgroup, 4 row groups
+__global__ __launch_bounds__(256) void colsum_kernel(const float* y, int64_t ld, int64_t rows, int64_t rows_per_chunk,
+                                                    float* part, int n) {
   __shared__ float red[4][64];
   const int c = blockIdx.x * 64 + (threadIdx.x & 63), rg = threadIdx.x >> 6;
+  const int64_t r0 = (int64_t)blockIdx.y * rows_per_chunk;
+  const int64_t r1 = r0 + rows_per_chunk < rows ? r0 + rows_per_chunk : rows;
   float s = 0.0f;
-  for (int64_t r = rg; r < rows; r += 4) s += y[r * ld + c];
+  for (int64_t r = r0 + rg; r < r1; r += 4) s += y[r * ld + c];
   red[rg][threadIdx.x & 63] = s;
   __syncthreads();
-  if (rg == 0) out[c] = (red[0][c & 63] + red[1][c & 63]) + (red[2][c & 63] + red[3][c & 63]);
+  if (rg == 0) part[(int64_t)blockIdx.y * n + c] = (red[0][c & 63] + red[1][c & 63]) + (red[2][c & 63] + red[3][c & 63]);
+}
+
+// out[i] = sum_z part[z][i], i < n (split-K and chunked column sums)
+__global__ __launch_bounds__(256) void sum_parts_kernel(const float* part, int nparts, int64_t n, float* out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float s = 0.0f;
+    for (int z = 0; z < nparts; ++z) s += part[(int64_t)z * n + i];
+    out[i] = s;
+  }
 }
 
 // Adam with bias correction folded into lr_t and eps_t; keeps the bf16 copy in step
@@ -260,7 +285,10 @@ struct tgr_mlp {
   uint16_t* dyh[2] = {nullptr, nullptr};
   float* part = nullptr;
   float* loss = nullptr;
+  float* kpart = nullptr;  // split-K / column-sum partials
+  int64_t kpart_n = 0;
   static constexpr int kLossBlocks = 256;
+  static constexpr int kMaxSplit = 32;
 
   void free_ws() {
     for (float* q : pre) (void)hipFree(q);
@@ -268,8 +296,10 @@ struct tgr_mlp {
     pre.clear();
     act.clear();
     for (void* q : {(void*)out, (void*)dyf[0], (void*)dyf[1], (void*)dyh[0], (void*)dyh[1], (void*)part,
-                    (void*)loss})
+                    (void*)loss, (void*)kpart})
       if (q) (void)hipFree(q);
+    kpart = nullptr;
+    kpart_n = 0;
     out = nullptr;
     dyf[0] = dyf[1] = nullptr;
     dyh[0] = dyh[1] = nullptr;
@@ -292,6 +322,11 @@ struct tgr_mlp {
     }
     HIP_TRY(hipMalloc(&part, sizeof(float) * kLossBlocks));
     HIP_TRY(hipMalloc(&loss, sizeof(float)));
+    // partials of the weight-gradient products (at most kMaxSplit slices of
+    // the largest weight matrix) followed by those of the bias column sums
+    // (at most kMaxSplit rows of the widest layer)
+    kpart_n = (int64_t)kMaxSplit * ((int64_t)wmax * wmax + wmax);
+    HIP_TRY(hipMalloc(&kpart, sizeof(float) * kpart_n));
     cap = rows;
     return 0;
   }
@@ -308,7 +343,9 @@ int launch_gemm(const GemmArgs& a, hipStream_t s) {
   // 16-byte loads: the other stride and the base must keep 8-element alignment
   if ((ak ? a.sam : a.sak) % 8 || (bk ? a.sbn : a.sbk) % 8 || ((uintptr_t)a.A & 15) || ((uintptr_t)a.B & 15))
     return fail(TGR_MLP_E_ARG, "gemm: operands must be 16-byte aligned with strides that are multiples of 8");
-  dim3 grid(a.N / kTile, a.M / kTile), block(256);
+  const int ks = a.ksplit > 1 ? a.ksplit : 1;
+  if (a.K % (ks * kKStep)) return fail(TGR_MLP_E_ARG, "gemm: K must split into multiples of 32");
+  dim3 grid(a.N / kTile, a.M / kTile, ks), block(256);
   if (ak && bk) gemm_bf16_kernel<true, true><<<grid, block, 0, s>>>(a);
   else if (ak) gemm_bf16_kernel<true, false><<<grid, block, 0, s>>>(a);
   else if (bk) gemm_bf16_kernel<false, true><<<grid, block, 0, s>>>(a);
@@ -321,6 +358,7 @@ GemmArgs plain_args() {
   GemmArgs a;
   memset(&a, 0, sizeof(a));
   a.act = kNone;
+  a.ksplit = 1;
   return a;
 }
 
@@ -385,12 +423,27 @@ int loss_and_grad(tgr_mlp* h, const uint16_t* x, const float* tgt, int64_t B, hi
     a.M = no;
     a.N = ni;
     a.K = (int)B;
-    a.out_f = h->g + h->offw[l];
     a.ldf = ni;
     a.n_valid = ni;
+    // split K (the batch) until the grid has >= 512 workgroups: the weight
+    // matrices alone give only (no / 64) x (ni / 64)
+    int ks = 1;
+    while (ks < tgr_mlp::kMaxSplit && (int64_t)(no / 64) * (ni / 64) * ks < 512 && B % (2 * ks * kKStep) == 0 &&
+           B / (2 * ks) >= 128)
+      ks *= 2;
+    a.ksplit = ks;
+    a.out_f = ks > 1 ? h->kpart : h->g + h->offw[l];
     if ((rc = launch_gemm(a, s))) return rc;
-    hipLaunchKernelGGL(colsum_kernel, dim3(no / 64), dim3(256), 0, s, h->dyf[cur], (int64_t)no, B,
-                       h->g + h->offb[l]);
+    if (ks > 1)
+      hipLaunchKernelGGL(sum_parts_kernel, dim3(256), dim3(256), 0, s, h->kpart, ks, (int64_t)no * ni,
+                         h->g + h->offw[l]);
+    // bias gradient: column sums of dY in row chunks, then the chunk sums
+    const int64_t nch = std::min<int64_t>(tgr_mlp::kMaxSplit, (B + 255) / 256);
+    const int64_t rpc = (B + nch - 1) / nch;
+    hipLaunchKernelGGL(colsum_kernel, dim3(no / 64, (unsigned)nch), dim3(256), 0, s, h->dyf[cur], (int64_t)no, B,
+                       rpc, h->kpart + (int64_t)tgr_mlp::kMaxSplit * no * ni, no);
+    hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(256), 0, s, h->kpart + (int64_t)tgr_mlp::kMaxSplit * no * ni,
+                       (int)nch, (int64_t)no, h->g + h->offb[l]);
     HIP_TRY(hipGetLastError());
     if (l == 0) break;
     // dY_prev = (dY W) * gelu'(pre_prev)
