@@ -172,7 +172,7 @@ def bench_surrogate(a, torch, fc, g, R_d, S0_d, S_d, HR_d, CS_d, dt, P, mech, sp
                            "plain_value": cells / t_rp, "guided_value": cells / t_rg, "speedup": t_rp / t_rg,
                            "accepted_fraction": sr["accepted"] / max(1, cells), "max_species_err": r_max,
                            "rms_species_err": r_rms, "max_T_err_K": rT},
-        "method": "per cell: network stage guesses -> 4 sub-steps of 3-stage L-stable SDIRK3(2), FP64 Newton, "
+        "method": "per cell: network stage guesses -> sub-steps of the 3-stage L-stable SDIRK3(2), FP64 Newton, "
                   "embedded error <= 1e-6 in Y, element-conserving projection, enthalpy-consistent T; rejected "
                   "cells by the plain FP64 step"}
 
@@ -259,6 +259,8 @@ def main():
     HR_d = fortran_empty(torch, tuple(R_d.shape), dev)
     CS_d = fortran_empty(torch, tuple(R_d.shape) + (3,), dev)
     fc = FiniteChem(mech, 1e-6, 1e-12, 10000)
+    from tiger_nga_amd import autotune
+    tuned = autotune.apply(fc, n)
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
     if a.kernel == "properties":
@@ -303,7 +305,8 @@ def main():
                 "synthetic seeded states (no case data ships with the reference)",
         "config": {"workload": "%s plain FP64 step, %s, dt %.1e s, P %.0f Pa, rtol 1e-6 atol 1e-12"
                                % (workload, cells_label, dt, P), "mechanism": mech, "path": "plain_fp64",
-                   "species": info["nsp"], "reactions": info["nrxn"], "rate_slots": info["nslot"]},
+                   "species": info["nsp"], "reactions": info["nrxn"], "rate_slots": info["nslot"],
+                   "tuned": tuned or "defaults (no tuning record for this device)"},
         "roofline": {"bound": "mfma", "compute_unit": "FP64 vector ALU (MI355X FP64 vector peak = FP64 matrix "
                                                        "peak = 78.6 TFLOP/s)",
                      "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",

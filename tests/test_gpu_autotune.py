@@ -1,0 +1,29 @@
+"""The autotuner measures its candidates on the GPU, records a valid choice
+for each knob, and applying the choices leaves the step's results unchanged
+(the grid and the batched right-hand-side mapping change scheduling only)."""
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_tune_and_apply_h2(tmp_path):
+    import torch
+    assert torch.cuda.is_available(), "GPU test without a HIP device"
+    from tiger_nga_amd import autotune, surrogate
+    from tiger_nga_amd.finitechem import FiniteChem
+    rec = autotune.tune("h2", cells=16 ** 3, reps=1, log=print)
+    assert rec["rhs_mapping"] in (0, 1) and rec["grid"] >= 0
+    assert set(rec["measured"]["rhs_ms"]) == {"lane", "wave"}
+    if "surr_nsub" in rec:
+        assert 1 <= rec["surr_nsub"] <= 8 and rec["surr_grade"] >= 1.0
+    cache = str(tmp_path / "tuning.json")
+    autotune.save(rec, 16 ** 3, cache)
+    fc = FiniteChem("h2")
+    R_d, S0, g, P = surrogate.block_states(fc, "h2", 16 ** 3, 5, 0)
+    ref = S0.clone()
+    fc.source_chemistry(R_d, ref, g, 1e-6, P, P)
+    done = autotune.apply(fc, 16 ** 3, cache)
+    assert done["grid"] == rec["grid"]
+    S = S0.clone()
+    fc.source_chemistry(R_d, S, g, 1e-6, P, P)
+    assert torch.equal(S, ref)

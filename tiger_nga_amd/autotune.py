@@ -1,0 +1,186 @@
+"""Autotuner: per mechanism and device, time the mapping and launch choices on
+a sample of the actual states and keep the fastest.
+
+Choices (all exposed as context options of include/tiger_chem.h):
+
+* ``rhs_mapping``  batched right-hand side (tgr_chem_rhs_device): one state
+  per lane (generated straight-line code) or one state per wavefront;
+* ``grid``         workgroups of the persistent step kernel: the resident
+  capacity (occupancy x CUs) or a fraction of it (fewer workgroups leave CUs
+  idle but reduce contention for the L2 and the work counter at small sizes);
+* ``surr_nsub`` / ``surr_grade``  sub-step schedule of the surrogate-guided
+  solve (when a trained network exists), chosen on relaxed states (after one
+  plain step, where the guided solve pays) for throughput among the schedules
+  whose max species error against the plain step stays <= 1e-6.
+
+Results are cached in ``tiger_nga_amd/tuning.json`` keyed by mechanism, device
+name and the sample size bucket; ``apply(fc)`` sets the cached choices on a
+FiniteChem.  Usage (on a GPU): ``python -m tiger_nga_amd.autotune --mech gri30``.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+from . import inputs
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CACHE = os.path.join(HERE, "tuning.json")
+WORKLOAD = {"h2": "h2", "gri30": "gri30", "dme39": "dme39"}
+
+
+def _device_name():
+    import torch
+    return torch.cuda.get_device_name(0)
+
+
+def _key(mech, cells):
+    return "%s|%s|%d" % (mech, _device_name(), 1 << max(0, int(cells - 1).bit_length()))
+
+
+def _time(torch, fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def tune(mech: str, cells: int = 32 ** 3, seed: int = 7, reps: int = 2, log=print):
+    """Measure every candidate on `cells` seeded states; returns the record."""
+    import torch
+    from . import surrogate
+    from .finitechem import FiniteChem
+    fc = FiniteChem(mech, 1e-6, 1e-12)
+    rec = {"mech": mech, "device": _device_name(), "cells": cells, "seed": seed, "measured": {}}
+    # batched right-hand side: lane vs wave per state
+    _, Y, T, rho, P, dt = inputs.states(WORKLOAD[mech], n=cells, seed=seed)
+    sol = torch.from_numpy(inputs.packed_state(Y, T, rho).T.copy()).cuda()
+    times = {}
+    for mapping in (1, 0):
+        fc.set_option("rhs_mapping", mapping)
+        times[mapping] = _time(torch, lambda: fc.compute_rhs(sol, P), reps * 5)
+    best_map = min(times, key=times.get)
+    rec["rhs_mapping"] = best_map
+    rec["measured"]["rhs_ms"] = {("lane" if k else "wave"): v * 1e3 for k, v in times.items()}
+    log("  rhs: lane %.3f ms, wave %.3f ms -> %s" % (times[1] * 1e3, times[0] * 1e3, "lane" if best_map else "wave"))
+    fc.set_option("rhs_mapping", best_map)
+    # persistent grid of the step kernel
+    R_d, S0, g, Pg = surrogate.block_states(fc, WORKLOAD[mech], cells, seed, 0)
+    S = S0.clone()
+    props = torch.cuda.get_device_properties(0)
+    n_cu = props.multi_processor_count
+    cand = {0: None}
+    for frac in (0.5, 0.75):
+        cand[max(1, int(n_cu * frac))] = frac
+
+    def step():
+        S.copy_(S0)
+        fc.source_chemistry(R_d, S, g, dt, Pg, Pg, sync=False)
+    gt = {}
+    for grid in cand:
+        fc.set_option("grid", grid)
+        gt[grid] = _time(torch, step, reps)
+        log("  step grid %s: %.2f ms" % (grid or "resident", gt[grid] * 1e3))
+    best_grid = min(gt, key=gt.get)
+    # keep the resident grid unless another is clearly faster (2 %)
+    if gt[best_grid] > 0.98 * gt[0]:
+        best_grid = 0
+    rec["grid"] = best_grid
+    rec["measured"]["step_ms"] = {str(k): v * 1e3 for k, v in gt.items()}
+    fc.set_option("grid", best_grid)
+    # surrogate schedule, measured on relaxed states (the block after one
+    # plain step): where the guided solve is used at all -- on freshly
+    # generated states the adaptive policy keeps the plain step
+    path = surrogate.model_path(mech)
+    if os.path.exists(path):
+        S.copy_(S0)
+        fc.source_chemistry(R_d, S, g, dt, Pg, Pg)
+        S0 = S.clone()
+        fc.source_chemistry(R_d, S, g, dt, Pg, Pg)
+        ref = S.clone()
+        nsp = fc.nsp
+        sur = surrogate.Surrogate.load(path)
+        sur.attach(fc)
+        fc.set_option("surr_probe", 0)
+        best = None
+        st_all = {}
+        for nsub, grade in ((2, 1.0), (4, 1.0), (6, 1.0), (8, 1.5)):
+            fc.set_option("surr_nsub", nsub)
+            fc.set_option("surr_grade", grade)
+
+            def sstep():
+                S.copy_(S0)
+                fc.source_chemistry_surrogate(R_d, S, g, dt, Pg, Pg, sync=False)
+            t = _time(torch, sstep, reps)
+            err = float((S[..., :nsp] - ref[..., :nsp]).abs().max())
+            st_all["%d/%.1f" % (nsub, grade)] = {"ms": t * 1e3, "max_species_err": err}
+            log("  surrogate nsub %d grade %.1f: %.2f ms, max |dY| %.2e" % (nsub, grade, t * 1e3, err))
+            if err <= 1e-6 and (best is None or t < best[0]):
+                best = (t, nsub, grade)
+        fc.set_surrogate(None, None, None, None)
+        rec["measured"]["surrogate"] = st_all
+        if best:
+            rec["surr_nsub"], rec["surr_grade"] = best[1], best[2]
+    fc.close()
+    return rec
+
+
+def load_cache(path=CACHE):
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return {}
+
+
+def save(rec, cells, path=CACHE):
+    cache = load_cache(path)
+    cache[_key(rec["mech"], cells)] = rec
+    with open(path, "w") as f:
+        json.dump(cache, f, indent=1, sort_keys=True)
+
+
+def lookup(mech: str, cells: int, path=CACHE):
+    """The cached record for this mechanism and device at the nearest size, or None."""
+    cache = load_cache(path)
+    dev = _device_name()
+    recs = [r for r in cache.values() if r.get("mech") == mech and r.get("device") == dev]
+    if not recs:
+        return None
+    return min(recs, key=lambda r: abs((r["cells"] or 1) - cells))
+
+
+def apply(fc, cells: int, path=CACHE) -> dict:
+    """Set the tuned choices on a FiniteChem; returns what was applied."""
+    rec = lookup(fc.mech, cells, path)
+    done = {}
+    if not rec:
+        return done
+    for k in ("rhs_mapping", "grid", "surr_nsub", "surr_grade"):
+        if k in rec:
+            fc.set_option(k, rec[k])
+            done[k] = rec[k]
+    return done
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--mech", nargs="*", default=["h2", "gri30", "dme39"])
+    ap.add_argument("--cells", type=int, default=32 ** 3)
+    ap.add_argument("--cache", default=CACHE)
+    a = ap.parse_args(argv)
+    for m in a.mech:
+        print("== %s" % m, flush=True)
+        rec = tune(m, a.cells, log=lambda s: print(s, flush=True))
+        save(rec, a.cells, a.cache)
+        print(json.dumps({k: v for k, v in rec.items() if k != "measured"}), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
