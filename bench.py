@@ -32,7 +32,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="gri30", choices=["h2", "gri30", "gri30_uneven", "dme39"])
+    ap.add_argument("--workload", default="gri30", choices=["h2", "gri30", "gri30_uneven", "dme39", "ethyred"])
     ap.add_argument("--cpu-cells", type=int, default=131072)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all usable host cores, at most 16")
     ap.add_argument("--no-cpu", action="store_true")
@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--data-file", default=None,
                     help="step the fields of an NGA data file instead of the synthetic workload "
                          "(needs --mech; no CPU baseline)")
-    ap.add_argument("--mech", default=None, choices=["h2", "gri30", "dme39"], help="mechanism of --data-file")
+    ap.add_argument("--mech", default=None, choices=["h2", "gri30", "dme39", "ethyred"], help="mechanism of --data-file")
     ap.add_argument("--pressure", type=float, default=101325.0, help="P_thermo for --data-file (Pa)")
     ap.add_argument("--dt", type=float, default=0.0, help="chemistry dt for --data-file (0: the file's dt)")
     ap.add_argument("--no-surrogate", action="store_true",
@@ -289,8 +289,11 @@ def main():
     F = flops.step_flops(info, stats)
     achieved = F / kernel_s
     cells = stats["cells"]
+    metric = "GRI-3.0 chem cell-steps/s, 1 GPU: surrogate-guided vs plain FP64 at err<=1e-6"
+    if mech != "gri30":
+        metric = metric.replace("GRI-3.0", {"h2": "HYDROGENF", "dme39": "DME39F", "ethyred": "EthyRedF"}[mech])
     out = {
-        "metric": "GRI-3.0 chem cell-steps/s, 1 GPU: surrogate-guided vs plain FP64 at err<=1e-6",
+        "metric": metric,
         "value": cells * a.steps / el,
         "unit": "cell-steps/s",
         "n_gpus": 1,

@@ -64,7 +64,8 @@ typedef struct {
  * options, work buffers.  Replaces finitechem_init (finitechem.f90:184-319:
  * GETNSPECIES, GETMOLARMASS, rtol/atol parser keys, defaults 1e-6 / 1e-12)
  * and the CVODE set-up of finitechem.f90:1014-1028 (FCVMALLOC, STOP_TIME,
- * MAX_NSTEPS = 10000, FCVDENSE).  mech: "h2" | "gri30" | "dme39". */
+ * MAX_NSTEPS = 10000, FCVDENSE).  mech: "h2" | "gri30" | "dme39" | "ethyred" (EthyRedF, the
+ * mechanism combustion/Makefile:3 links by default). */
 int tgr_chem_create(const char *mech, double rtol, double atol, int max_steps, tgr_chem **out);
 
 /* Release the context (FCVFREE, finitechem.f90:1068). */

@@ -18,6 +18,7 @@
 #include "../tiger_nga_amd/csrc/mech_h2.h"
 #include "../tiger_nga_amd/csrc/mech_gri30.h"
 #include "../tiger_nga_amd/csrc/mech_dme39.h"
+#include "../tiger_nga_amd/csrc/mech_ethyred.h"
 
 /* unsuffixed Fortran literals are binary32 values widened to double */
 #define F4(x) ((double)(x##f))
@@ -27,6 +28,7 @@ const tgr_mech_tables *orc_tables(int mech) {
     case ORC_H2: return &h2_tables;
     case ORC_GRI30: return &gri30_tables;
     case ORC_DME39: return &dme39_tables;
+    case ORC_ETHYRED: return &ethyred_tables;
     default: return 0;
   }
 }

@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-enum { ORC_H2 = 0, ORC_GRI30 = 1, ORC_DME39 = 2 };
+enum { ORC_H2 = 0, ORC_GRI30 = 1, ORC_DME39 = 2, ORC_ETHYRED = 3 };
 
 typedef struct {
   int nsp, nrxn, nslot, n_third;

@@ -13,7 +13,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
-MECH_ID = {"h2": 0, "gri30": 1, "dme39": 2}
+MECH_ID = {"h2": 0, "gri30": 1, "dme39": 2, "ethyred": 3}
 
 
 class Opts(ctypes.Structure):

@@ -13,9 +13,11 @@
 
 #include "bdf_wave.h"
 #include "lane_rhs_dme39.h"
+#include "lane_rhs_ethyred.h"
 #include "lane_rhs_gri30.h"
 #include "lane_rhs_h2.h"
 #include "mech_dme39.h"
+#include "mech_ethyred.h"
 #include "mech_gri30.h"
 #include "mech_h2.h"
 #include "tables_host.h"
@@ -26,6 +28,7 @@ const tgr_mech_tables* find_mech(const char* name) {
   if (!std::strcmp(name, "h2")) return &h2_tables;
   if (!std::strcmp(name, "gri30")) return &gri30_tables;
   if (!std::strcmp(name, "dme39")) return &dme39_tables;
+  if (!std::strcmp(name, "ethyred")) return &ethyred_tables;
   return nullptr;
 }
 
@@ -134,6 +137,7 @@ extern "C" int emu_step(const char* mech, long ncell, double* rho, double* sc, d
     case 10: run<10>(p, counters); break;
     case 37: run<37>(p, counters); break;
     case 40: run<40>(p, counters); break;
+    case 46: run<46>(p, counters); break;
     default: return -3;
   }
   return 0;
@@ -157,6 +161,7 @@ extern "C" int emu_lane_rhs(const char* mech, long n, double p_gp, const double*
     case 10: run_n(std::integral_constant<int, 10>()); break;
     case 37: run_n(std::integral_constant<int, 37>()); break;
     case 40: run_n(std::integral_constant<int, 40>()); break;
+    case 46: run_n(std::integral_constant<int, 46>()); break;
     default: return -3;
   }
   return 0;

@@ -26,7 +26,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.abspath(os.path.join(HERE, "..", ".."))
 REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libref_%s.so")
-MECHS = ("h2", "gri30", "dme39")
+MECHS = ("h2", "gri30", "dme39", "ethyred")
 NSTATE = 24
 NTHERMO = 41
 
@@ -61,8 +61,11 @@ def states(nsp, seed):
     return T, P, C
 
 
-def main():
+def main(argv=None):
+    only = set(argv or sys.argv[1:])
     for i, mech in enumerate(MECHS):
+        if only and mech not in only:
+            continue
         lib, nsp, nslot = load(mech)
         # third-body sums: the mechanism declares M(n) locally; probe size from the header
         nm = 64

@@ -20,12 +20,14 @@ def _bf(torch, a):
     return torch.from_numpy(np.asarray(a, dtype=np.float32)).to(torch.bfloat16).cuda()
 
 
-@pytest.mark.parametrize("a_kcontig,b_kcontig", [(True, True), (True, False), (False, True), (False, False)])
-def test_gemm_layouts(torch_dev, a_kcontig, b_kcontig):
+@pytest.mark.parametrize("a_kcontig,b_kcontig,M,N,K", [
+    (True, True, 128, 192, 96), (True, True, 256, 256, 64), (True, True, 64, 128, 32), (True, False, 128, 192, 96),
+    (False, True, 128, 192, 96), (False, False, 128, 192, 96)])
+def test_gemm_layouts(torch_dev, a_kcontig, b_kcontig, M, N, K):
+    """The product kernel in each operand layout and at several tile counts."""
     torch = torch_dev
     from tiger_nga_amd.mlp import MLP
     net = MLP(8, 8, 64, 1)
-    M, N, K = 128, 192, 96
     rng = np.random.default_rng(7)
     A = rng.standard_normal((M, K))
     Bm = rng.standard_normal((K, N))

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
-MECHS = ("h2", "gri30", "dme39")
+MECHS = ("h2", "gri30", "dme39", "ethyred")
 
 
 def _gold(mech):

@@ -29,7 +29,7 @@ from . import inputs
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuning.json")
-WORKLOAD = {"h2": "h2", "gri30": "gri30", "dme39": "dme39"}
+WORKLOAD = {"h2": "h2", "gri30": "gri30", "dme39": "dme39", "ethyred": "ethyred"}
 
 
 def _device_name():

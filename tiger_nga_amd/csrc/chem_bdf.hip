@@ -10,6 +10,7 @@
 #include "bdf_wave.h"
 #include "chem_kernels.h"
 #include "lane_rhs_dme39.h"
+#include "lane_rhs_ethyred.h"
 #include "lane_rhs_gri30.h"
 #include "lane_rhs_h2.h"
 
@@ -201,6 +202,7 @@ extern "C" int tgr_step_occupancy(int neq) {
     case 10: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_kernel<10>, kBlock, 0); break;
     case 37: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_kernel<37>, kBlock, 0); break;
     case 40: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_kernel<40>, kBlock, 0); break;
+    case 46: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_kernel<46>, kBlock, 0); break;
     default: break;
   }
   return e == hipSuccess ? nb : 0;
@@ -213,6 +215,7 @@ extern "C" int tgr_launch_step(const StepParams* p, int neq, int grid, hipStream
     case 10: hipLaunchKernelGGL(step_kernel<10>, dim3(grid), dim3(kBlock), 0, s, *p); break;
     case 37: hipLaunchKernelGGL(step_kernel<37>, dim3(grid), dim3(kBlock), 0, s, *p); break;
     case 40: hipLaunchKernelGGL(step_kernel<40>, dim3(grid), dim3(kBlock), 0, s, *p); break;
+    case 46: hipLaunchKernelGGL(step_kernel<46>, dim3(grid), dim3(kBlock), 0, s, *p); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
@@ -226,6 +229,7 @@ extern "C" int tgr_launch_prodrates(const DevMech* m, int64_t n, const double* C
     case 10: hipLaunchKernelGGL(prodrates_kernel<10>, g, b, 0, s, *m, n, C, T, P, cdot); break;
     case 37: hipLaunchKernelGGL(prodrates_kernel<37>, g, b, 0, s, *m, n, C, T, P, cdot); break;
     case 40: hipLaunchKernelGGL(prodrates_kernel<40>, g, b, 0, s, *m, n, C, T, P, cdot); break;
+    case 46: hipLaunchKernelGGL(prodrates_kernel<46>, g, b, 0, s, *m, n, C, T, P, cdot); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
@@ -238,6 +242,7 @@ extern "C" int tgr_launch_rhs_lanes(int neq, int64_t n, double p_gp, const doubl
     case 10: hipLaunchKernelGGL(rhs_lanes_kernel<10>, grid, dim3(64), 0, s, n, p_gp, y, f); break;
     case 37: hipLaunchKernelGGL(rhs_lanes_kernel<37>, grid, dim3(64), 0, s, n, p_gp, y, f); break;
     case 40: hipLaunchKernelGGL(rhs_lanes_kernel<40>, grid, dim3(64), 0, s, n, p_gp, y, f); break;
+    case 46: hipLaunchKernelGGL(rhs_lanes_kernel<46>, grid, dim3(64), 0, s, n, p_gp, y, f); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
@@ -250,6 +255,7 @@ extern "C" int tgr_launch_rhs(const DevMech* m, int64_t n, double p_gp, const do
     case 10: hipLaunchKernelGGL(rhs_kernel<10>, g, b, 0, s, *m, n, p_gp, y, f); break;
     case 37: hipLaunchKernelGGL(rhs_kernel<37>, g, b, 0, s, *m, n, p_gp, y, f); break;
     case 40: hipLaunchKernelGGL(rhs_kernel<40>, g, b, 0, s, *m, n, p_gp, y, f); break;
+    case 46: hipLaunchKernelGGL(rhs_kernel<46>, g, b, 0, s, *m, n, p_gp, y, f); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
@@ -263,6 +269,7 @@ extern "C" int tgr_launch_temperature(const DevMech* m, int64_t n, const double*
     case 10: hipLaunchKernelGGL(temperature_kernel<10>, grid, dim3(64), 0, s, *m, n, Y, hmix, T0, T, iters); break;
     case 37: hipLaunchKernelGGL(temperature_kernel<37>, grid, dim3(64), 0, s, *m, n, Y, hmix, T0, T, iters); break;
     case 40: hipLaunchKernelGGL(temperature_kernel<40>, grid, dim3(64), 0, s, *m, n, Y, hmix, T0, T, iters); break;
+    case 46: hipLaunchKernelGGL(temperature_kernel<46>, grid, dim3(64), 0, s, *m, n, Y, hmix, T0, T, iters); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;

@@ -14,6 +14,7 @@
 #include "chem_kernels.h"
 #include "tables_host.h"
 #include "mech_dme39.h"
+#include "mech_ethyred.h"
 #include "mech_gri30.h"
 #include "mech_h2.h"
 
@@ -41,6 +42,7 @@ const tgr_mech_tables* find_mech(const char* name) {
   if (!strcmp(name, "h2")) return &h2_tables;
   if (!strcmp(name, "gri30")) return &gri30_tables;
   if (!strcmp(name, "dme39")) return &dme39_tables;
+  if (!strcmp(name, "ethyred")) return &ethyred_tables;
   return nullptr;
 }
 
@@ -287,7 +289,7 @@ int tgr_chem_create(const char* mech, double rtol, double atol, int max_steps, t
   if (!t) return fail(TGR_E_MECH, std::string("unknown mechanism: ") + (mech ? mech : "(null)"));
   if (!(rtol > 0.0) || !(atol > 0.0)) return fail(TGR_E_ARG, "tolerances must be positive");
   const int neq = t->nsp + 1;
-  if (neq != 10 && neq != 37 && neq != 40) return fail(TGR_E_MECH, "no kernel instance for this size");
+  if (neq != 10 && neq != 37 && neq != 40 && neq != 46) return fail(TGR_E_MECH, "no kernel instance for this size");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(TGR_E_HIP, "no HIP device");
   tgr_chem* c = new tgr_chem();

@@ -69,6 +69,10 @@ template <>
 struct MechT<40> {  // dme39 (DME39F)
   static constexpr int nrxn = 175, nfo = 11, nthird = 20, gmax = 15, maxe = 6;
 };
+template <>
+struct MechT<46> {  // ethyred (EthyRedF, the mechanism combustion/Makefile:3 links by default)
+  static constexpr int nrxn = 193, nfo = 18, nthird = 7, gmax = 18, maxe = 6;
+};
 
 // Byte layout of the table block for a mechanism shape (rows of 64 lanes:
 // a double row is 512 bytes, an int row 256).

@@ -102,6 +102,7 @@ extern "C" int tgr_launch_props(const PropsParams* p, int neq, hipStream_t s) {
     case 10: hipLaunchKernelGGL(props_kernel<10>, grid, block, 0, s, *p); break;
     case 37: hipLaunchKernelGGL(props_kernel<37>, grid, block, 0, s, *p); break;
     case 40: hipLaunchKernelGGL(props_kernel<40>, grid, block, 0, s, *p); break;
+    case 46: hipLaunchKernelGGL(props_kernel<46>, grid, block, 0, s, *p); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;

@@ -256,6 +256,7 @@ extern "C" int tgr_surr_occupancy(int neq) {
     case 10: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, surr_kernel<10>, kSurrBlock, 0); break;
     case 37: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, surr_kernel<37>, kSurrBlock, 0); break;
     case 40: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, surr_kernel<40>, kSurrBlock, 0); break;
+    case 46: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, surr_kernel<46>, kSurrBlock, 0); break;
     default: break;
   }
   return e == hipSuccess ? nb : 0;
@@ -269,6 +270,7 @@ extern "C" int tgr_launch_surr_features(const StepParams* p, int neq, int64_t ro
     case 10: hipLaunchKernelGGL(surr_features_kernel<10>, grid, block, 0, s, *p, rows, in_pad, shift, scale, log_dt, X); break;
     case 37: hipLaunchKernelGGL(surr_features_kernel<37>, grid, block, 0, s, *p, rows, in_pad, shift, scale, log_dt, X); break;
     case 40: hipLaunchKernelGGL(surr_features_kernel<40>, grid, block, 0, s, *p, rows, in_pad, shift, scale, log_dt, X); break;
+    case 46: hipLaunchKernelGGL(surr_features_kernel<46>, grid, block, 0, s, *p, rows, in_pad, shift, scale, log_dt, X); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
@@ -287,6 +289,7 @@ extern "C" int tgr_launch_surr(const SurrParams* P, int neq, int grid, hipStream
     case 10: hipLaunchKernelGGL(surr_kernel<10>, dim3(grid), dim3(kSurrBlock), 0, s, *P); break;
     case 37: hipLaunchKernelGGL(surr_kernel<37>, dim3(grid), dim3(kSurrBlock), 0, s, *P); break;
     case 40: hipLaunchKernelGGL(surr_kernel<40>, dim3(grid), dim3(kSurrBlock), 0, s, *P); break;
+    case 46: hipLaunchKernelGGL(surr_kernel<46>, dim3(grid), dim3(kSurrBlock), 0, s, *P); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;

@@ -15,6 +15,8 @@ static int seterr(std::string* err, const char* msg) {
   return -1;
 }
 
+static int seterr(std::string* err, const std::string& msg) { return seterr(err, msg.c_str()); }
+
 // Builds the table block (layout tgr::TabLayout for the mechanism's
 // compile-time shape): reactions re-ordered fall-off first (so only the first
 // 64-reaction round of a wave diverges), species padded to 64 lanes, the
@@ -27,6 +29,7 @@ int build_table_blob(const tgr_mech_tables* t, std::vector<char>& blob, tgr_mech
     case 10: L = tgr::tab_layout<10>(); break;
     case 37: L = tgr::tab_layout<37>(); break;
     case 40: L = tgr::tab_layout<40>(); break;
+    case 46: L = tgr::tab_layout<46>(); break;
     default: return seterr(err, "no kernel instance for this mechanism size");
   }
   std::vector<int> order;
@@ -58,7 +61,9 @@ int build_table_blob(const tgr_mech_tables* t, std::vector<char>& blob, tgr_mech
     if (lanes <= 64) break;
   }
   if (nr != L.nrxn || nfo != L.nfo || t->n_third != L.nthird || S != L.gmax || maxe != L.maxe)
-    return seterr(err, "mechanism tables do not match the compiled kernel shape");
+    return seterr(err, "mechanism tables do not match the compiled kernel shape (nrxn " + std::to_string(nr) +
+                           ", nfo " + std::to_string(nfo) + ", nthird " + std::to_string(t->n_third) + ", gmax " +
+                           std::to_string(S) + ", maxe " + std::to_string(maxe) + ")");
   if (nsp > kZero || t->n_third > kZero) return seterr(err, "too many species or third-body sums for 64 lanes");
   if (nr >= L.rounds * 64) return seterr(err, "no padding slot in the last reaction round");
   const int qzero = L.rounds * 64 - 1;  // rate-of-progress slot of a padding reaction (always 0)

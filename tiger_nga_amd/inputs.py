@@ -18,6 +18,9 @@ seeded numpy PCG64 stream:
               interleaved.
 * ``dme39``   39 species, as gri30 with CH3OCH3 fuel (CH3OCH3 + 3 O2 ->
               2 CO2 + 3 H2O at full conversion), T~U(600, 2200) K.
+* ``ethyred`` 45 species (EthyRedF, the mechanism the reference links by
+              default), as gri30 with C2H4 fuel (C2H4 + 3 O2 -> 2 CO2 +
+              2 H2O), 64^3 cells, 1 atm, dt 5e-7 s, T~U(800, 2400) K.
 
 Density follows the EOS of finitechem_density (combustion/finitechem.f90:
 324-345): rho = P W_mix / (R T), R = 8314.34 J/kmol/K.
@@ -37,6 +40,7 @@ WORKLOADS = {
     "gri30": ("gri30", 64 ** 3, P_ATM, 5e-7),
     "gri30_uneven": ("gri30", 128 ** 3, P_ATM, 5e-7),
     "dme39": ("dme39", 128 ** 3, 10 * P_ATM, 2e-7),
+    "ethyred": ("ethyred", 64 ** 3, P_ATM, 5e-7),
 }
 
 
@@ -102,8 +106,8 @@ def states(workload: str, n: int | None = None, seed: int = 0):
         Y = _minors(mech, Y, 1e-6, 1e-3, ("N2", "O2", "H2", "H2O"), rng)
         T = rng.uniform(900.0, 2200.0, n)
     else:
-        fuel = "CH4" if mech == "gri30" else "CH3OCH3"
-        nC, nH, o2 = (1, 4, 2.0) if fuel == "CH4" else (2, 6, 3.0)
+        fuel = {"gri30": "CH4", "dme39": "CH3OCH3", "ethyred": "C2H4"}[mech]
+        nC, nH, o2 = {"CH4": (1, 4, 2.0), "CH3OCH3": (2, 6, 3.0), "C2H4": (2, 4, 3.0)}[fuel]
         Y = _mix(mech, n, 0.15, fuel, rng)
         if workload == "gri30_uneven":
             u = rng.random(n)
@@ -112,7 +116,7 @@ def states(workload: str, n: int | None = None, seed: int = 0):
             T = np.choose(kind, [rng.uniform(1800, 2300, n), rng.uniform(1000, 1300, n), rng.uniform(300, 600, n)])
         else:
             c = rng.uniform(0.0, 1.0, n)
-            T = rng.uniform(800.0, 2400.0, n) if mech == "gri30" else rng.uniform(600.0, 2200.0, n)
+            T = rng.uniform(600.0, 2200.0, n) if mech == "dme39" else rng.uniform(800.0, 2400.0, n)
         Y = _burn(mech, Y, c, fuel, nC, nH, o2, 0.1, 0.05)
         Y = _minors(mech, Y, 1e-8, 1e-3, ("N2", "O2", fuel, "CO2", "H2O", "CO", "H2"), rng)
     W = meta.molar_mass(mech)

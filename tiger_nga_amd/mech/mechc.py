@@ -41,6 +41,7 @@ MECHANISMS = {
     "h2": ("combustion/HYDROGENF.f90", "combustion/HYDROGENF90.h"),
     "gri30": ("combustion/grimech30F.f90", "combustion/grimech30F90.h"),
     "dme39": ("combustion/DME39F.f90", "combustion/DME39F90.h"),
+    "ethyred": ("combustion/EthyRedF.f90", "combustion/EthyRedF90.h"),
 }
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -284,6 +285,12 @@ class Parser:
             e = self.expr()
             self.take(")")
             return e
+        if v in ("+", "-"):
+            # a signed operand after an operator, e.g. TEMP / -10210
+            # (EthyRedF.f90 Troe centres): the common compiler extension
+            # binds the sign to the operand that follows
+            e = self.power()
+            return Neg(e) if v == "-" else e
         raise SyntaxError("unexpected %r" % v)
 
 
@@ -647,10 +654,12 @@ def compile_mechanism(name: str, ref_root: str) -> dict:
     if nreac_decl is not None and nreac_decl != nslot:
         raise MechError("GETNREACTIONS=%d but %d slots" % (nreac_decl, nslot))
 
-    # slot names from the header: R<n>F / R<n>B / R<n>
+    # slot names from the header: R<n>F / R<n>B / R<n> (and RG<n>...)
     slotname = {}
     for k, v in params.items():
-        if re.fullmatch(r"r\d+[fb]?", k):
+        # every R-named parameter below REND is a rate slot: R12, R2F / R2B,
+        # and free-form names such as RG04, RLPP009F, RIC67 (EthyRedF90.h)
+        if k.startswith("r") and k != "rend" and 1 <= v < params.get("rend", 1 << 30):
             slotname[v - 1] = k.upper()
     # pair forward/backward slots into reactions
     reactions = []

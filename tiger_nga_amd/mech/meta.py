@@ -14,13 +14,16 @@ import re
 import numpy as np
 
 _CSRC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "csrc")
-MECHS = ("h2", "gri30", "dme39")
+MECHS = ("h2", "gri30", "dme39", "ethyred")
 ELEMENTS = ("C", "H", "O", "N", "AR")
 
 ALIASES = {
     "CH2*-CH2": "CH2", "CH2X": "CH2",
     "RO2": "CH3OCH2O2", "ROH": "CH3OCH2OH", "QOOH": "CH2OCH2O2H",
     "O2QOOH": "O2CH2OCH2O2H", "HO2QHO": "HO2CH2OCHO",
+    # EthyRedF names cut to 8 characters by GETSPECIESNAMES (EthyRedF.f90:1652-1659);
+    # the full names of EthyRedF90.h give the formulas (A1C2H2XC8H7 etc.)
+    "A1C2H2XC": "C8H7", "A1CH2XC7": "C7H7", "A1C2HYXC": "C8H5", "A2XXC10H": "C10H7", "A1C2HXC8": "C8H6",
 }
 
 
@@ -47,6 +50,10 @@ def composition(name: str) -> dict:
     if name == "AR":
         return {"AR": 1}
     f = ALIASES.get(name, name)
+    if f == name and "X" in name:
+        # FlameMaster isomer / structure prefixes: TXCH2, SXCH2, IXC3H7,
+        # A1XC6H6, A1XXC6H5: the formula follows the last X
+        f = name.rsplit("X", 1)[1]
     out = {}
     for el, n in re.findall(r"(C|H|O|N)(\d*)", f):
         out[el] = out.get(el, 0) + (int(n) if n else 1)

@@ -47,6 +47,7 @@ TRAIN_SETS = {
     "gri30": [("gri30", 64 ** 3, 0), ("gri30_uneven", 64 ** 3, 0), ("gri30", 64 ** 3, 1),
               ("gri30_uneven", 64 ** 3, 1)],
     "dme39": [("dme39", 64 ** 3, 0), ("dme39", 64 ** 3, 1)],
+    "ethyred": [("ethyred", 64 ** 3, 0), ("ethyred", 64 ** 3, 1)],
 }
 
 
