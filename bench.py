@@ -68,14 +68,28 @@ def cpu_baseline(workload, ncell, nthreads):
     t0 = time.perf_counter()
     pyoracle.integrate_batch(mech, sol, dt, P, rtol=1e-6, atol=1e-12, nthreads=nthreads)
     el = time.perf_counter() - t0
+    result = sol.copy()
     n1 = max(16, ncell // 64)
     t0 = time.perf_counter()
     pyoracle.integrate_batch(mech, sol[:n1], dt, P, rtol=1e-6, atol=1e-12, nthreads=1)
     el1 = time.perf_counter() - t0
-    return {"value": ncell / el, "unit": "cell-steps/s", "cores": nthreads, "kind": "port",
-            "sample": "first %d cells of the %s workload (same seeded states), C oracle BDF at rtol 1e-6, "
-                      "%d threads x serial cell loop; %.2f s wall" % (ncell, workload, nthreads, el),
-            "single_core_value": n1 / el1}
+    rec = {"value": ncell / el, "unit": "cell-steps/s", "cores": nthreads, "kind": "port",
+           "sample": "first %d cells of the %s workload (same seeded states), C oracle BDF at rtol 1e-6, "
+                     "%d threads x serial cell loop; %.2f s wall" % (ncell, workload, nthreads, el),
+           "single_core_value": n1 / el1}
+    # oracle result as mass fractions and T, for the error of the GPU paths
+    rho = result[:, :-1].sum(axis=1)
+    return rec, np.concatenate([result[:, :-1] / rho[:, None], (result[:, -1] / rho)[:, None]], axis=1)
+
+
+def errors_vs_oracle(torch, S, nsp, ref):
+    """Max / RMS mass-fraction error and max T error of the first len(ref)
+    interior cells of a Fortran-strided (nx, ny, nz, nscalar) result."""
+    n = ref.shape[0]
+    got = torch.stack([S[..., k].permute(2, 1, 0).reshape(-1)[:n] for k in range(nsp + 1)], dim=1).cpu().numpy()
+    dY = abs(got[:, :nsp] - ref[:, :nsp])
+    return {"cells": n, "max_species_err": float(dY.max()), "rms_species_err": float((dY ** 2).mean() ** 0.5),
+            "max_T_err_K": float(abs(got[:, nsp] - ref[:, nsp]).max())}
 
 
 def traffic_record(workload):
@@ -134,6 +148,7 @@ def bench_surrogate(a, torch, fc, g, R_d, S0_d, S_d, HR_d, CS_d, dt, P, mech, sp
     S_d.copy_(S0_d)
     _, ss = fc.source_chemistry_surrogate(R_d, S_d, g, dt, P, P, HR_gp=HR_d, chemsrc_gp=CS_d, stream=sp)
     e_max, e_rms, eT = errors(Sp)
+    state = S_d.clone()  # the guided result on the benchmark states, for the oracle comparison
     fc.set_option("surr_guess", 0)
     S_d.copy_(S0_d)
     _, s0 = fc.source_chemistry_surrogate(R_d, S_d, g, dt, P, P, stream=sp)
@@ -156,6 +171,7 @@ def bench_surrogate(a, torch, fc, g, R_d, S0_d, S_d, HR_d, CS_d, dt, P, mech, sp
     fc.set_option("surr_probe", 8)
     fc.set_surrogate(None, None, None, None)
     return {
+        "_state": state,
         "value": cells / t_adapt, "ms_per_step": t_adapt * 1e3, "model": os.path.relpath(path, ROOT),
         "policy": "adaptive: guided solve while its accepted fraction >= 0.15, else plain step with a guided "
                   "probe every 8 calls",
@@ -319,8 +335,11 @@ def main():
         "solver": {k: stats[k] for k in ("nst", "nfe", "nfe_dq", "nje", "nsetups", "nni", "netf", "ncfn", "nlu",
                                          "failed")},
     }
+    S_plain = S_d.clone()
     sur = bench_surrogate(a, torch, fc, g, R_d, S0_d, S_d, HR_d, CS_d, dt, P, mech, sp, stream, cells)
+    sur_state = None
     if sur is not None:
+        sur_state = sur.pop("_state")
         out["surrogate"] = sur
         # the metric compares the surrogate-guided step with the plain FP64 one
         # at max species error <= 1e-6: the guided path is the headline when
@@ -336,7 +355,11 @@ def main():
         out["roofline"]["traffic_source"] = tr["source"]
     if not a.no_cpu:
         nt = a.cpu_threads or min(16, len(os.sched_getaffinity(0)))
-        out["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_cells, nt)
+        out["cpu_baseline"], ref = cpu_baseline(a.workload, a.cpu_cells, nt)
+        # both GPU paths against the FP64 oracle on the oracle's cells
+        out["plain_vs_oracle"] = errors_vs_oracle(torch, S_plain, fc.nsp, ref)
+        if sur is not None:
+            out["surrogate"]["vs_oracle"] = errors_vs_oracle(torch, sur_state, fc.nsp, ref)
     print(json.dumps(out), flush=True)
     return 0
 
