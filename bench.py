@@ -63,12 +63,14 @@ def cpu_baseline(workload, ncell, nthreads):
 
     from oracle import pyoracle
     from tiger_nga_amd import inputs
-    mech, Y, T, rho, P, dt = inputs.states(workload, n=ncell, seed=0)
-    sol = inputs.packed_state(Y, T, rho)
+    # the benchmark's own states (the generator draws per field over all
+    # cells, so a shorter run of it would give other states): its first cells
+    mech, Y, T, rho, P, dt = inputs.states(workload, seed=0)
+    ncell = min(ncell, Y.shape[0])
+    sol = inputs.packed_state(Y[:ncell], T[:ncell], rho[:ncell])
     t0 = time.perf_counter()
-    pyoracle.integrate_batch(mech, sol, dt, P, rtol=1e-6, atol=1e-12, nthreads=nthreads)
+    result = pyoracle.integrate_batch(mech, sol, dt, P, rtol=1e-6, atol=1e-12, nthreads=nthreads)
     el = time.perf_counter() - t0
-    result = sol.copy()
     n1 = max(16, ncell // 64)
     t0 = time.perf_counter()
     pyoracle.integrate_batch(mech, sol[:n1], dt, P, rtol=1e-6, atol=1e-12, nthreads=1)
