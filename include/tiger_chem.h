@@ -183,6 +183,18 @@ int tgr_chem_source_chemistry_surrogate_device(tgr_chem *ctx, double dt, double 
                                                double *HR_gp, double *chemsrc_gp, void *stream, tgr_stats *stats,
                                                tgr_surrogate_stats *sstats);
 
+/* The same on HOST arrays (staged through device buffers; synchronous), the
+ * counterpart of tgr_chem_source_chemistry for a Fortran caller. */
+int tgr_chem_source_chemistry_surrogate(tgr_chem *ctx, double dt, double p_thermo, double p_thermo_old,
+                                        const tgr_grid *g, double *RHO_, double *SC_, const int *mask, double *HR_gp,
+                                        double *chemsrc_gp, tgr_stats *stats, tgr_surrogate_stats *sstats);
+
+/* Load a trained network file (tiger_nga_amd/models/<mech>.bin: "TGRMLP01",
+ * int32 nin nout hidden layers, int64 nparam, float32 in_shift[nin]
+ * in_scale[nin], float64 out_scale[nout], float32 params[nparam]) and attach
+ * it; the context owns that network and frees it in tgr_chem_destroy. */
+int tgr_chem_load_surrogate(tgr_chem *ctx, const char *path);
+
 /* Real-valued knobs of the surrogate step: "surr_tol_y" (default 1e-6),
  * "surr_tol_t" (K, default 1e-2), "surr_newton_tol" (Newton stops when the
  * correction is below this fraction of the acceptance tolerance, default 0.1),

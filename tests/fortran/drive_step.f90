@@ -1,6 +1,7 @@
 ! Fortran caller of the chemistry step through the ISO_C_BINDING module, as a
 ! Tiger_NGA rank would call it in place of finitechem_source_chemistry.
-! Usage: drive_step <mech> <in.bin> <out.bin>
+! Usage: drive_step <mech> <in.bin> <out.bin> [network.bin]
+!   with a network file the surrogate-guided step is called instead
 !   in.bin  (stream): int32 nx ny nz nover nscalar isc_sc isc_T; float64 dt P rtol atol;
 !           RHO_(nxo,nyo,nzo); SC_(nxo,nyo,nzo,nscalar); int32 mask(nxo,nyo)
 !   out.bin (stream): int32 ierr; SC_; HR_gp; chemsrc_gp(...,3); int64 stats(11)
@@ -9,7 +10,7 @@ program drive_step
   use tiger_chem
   implicit none
   character(len=64) :: mech
-  character(len=512) :: fin, fout
+  character(len=512) :: fin, fout, fnet
   integer(c_int) :: hdr(7)
   integer :: nx, ny, nz, nover, nscalar, ierr, u
   real(c_double) :: dt, P, rtol, atol
@@ -21,6 +22,8 @@ program drive_step
   call get_command_argument(1, mech)
   call get_command_argument(2, fin)
   call get_command_argument(3, fout)
+  fnet = ''
+  if (command_argument_count() >= 4) call get_command_argument(4, fnet)
   open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
   read(u) hdr
   nx = hdr(1); ny = hdr(2); nz = hdr(3); nover = hdr(4); nscalar = hdr(5)
@@ -38,7 +41,13 @@ program drive_step
   g = tgr_grid(1, nx, 1, ny, 1, nz, 1-nover, nx+nover, 1-nover, ny+nover, 1-nover, nz+nover, &
        1-nover, nx+nover, 1-nover, ny+nover, nscalar, hdr(6), hdr(7))
   call tiger_chem_init(mech, rtol, atol, ierr)
-  if (ierr == 0) call tiger_chem_source_chemistry(RHO_, SC_, g, dt, P, P, ierr, mask, HR_gp, chemsrc_gp, st)
+  if (len_trim(fnet) > 0) then
+     if (ierr == 0) call tiger_chem_load_surrogate(fnet, ierr)
+     if (ierr == 0) call tiger_chem_source_chemistry_surrogate(RHO_, SC_, g, dt, P, P, ierr, mask, HR_gp, &
+          chemsrc_gp, st)
+  else
+     if (ierr == 0) call tiger_chem_source_chemistry(RHO_, SC_, g, dt, P, P, ierr, mask, HR_gp, chemsrc_gp, st)
+  end if
   if (ierr /= 0) then
      write(*, '(a, i0, 2a)') 'tiger_chem error ', ierr, ': ', trim(tiger_chem_message())
      st = tgr_stats(0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0.0_c_double)

@@ -84,3 +84,34 @@ def test_fortran_caller_matches_oracle(tmp_path, oracle):
     assert np.max(np.abs(Yg - Yr) / np.maximum(np.abs(Yr), 1e-12)) < 1e-8
     assert np.max(np.abs(sc[..., g.isc_T - 1] - ref[..., g.isc_T - 1])) < 1e-6
     np.testing.assert_allclose(hr, hr_ref, rtol=1e-6, atol=1e-6 * np.abs(hr_ref).max())
+
+
+@pytest.mark.gpu
+def test_fortran_caller_surrogate_path(tmp_path):
+    """The surrogate-guided step from Fortran (network file loaded through the
+    C ABI) gives the same fields as the same call from Python on host arrays,
+    and each cell is within 1e-6 of the plain step."""
+    _need_driver()
+    from tiger_nga_amd import surrogate
+    from tiger_nga_amd.finitechem import FiniteChem
+    net = surrogate.model_path("gri30")
+    mech, RHO_, SC_, mask, g, dt, P = _write_case(tmp_path / "in.bin", "gri30", 4, 4, 3, 1, 1e-6, 1e-12)
+    print("fortran caller, surrogate path: gri30 4x4x3 + 1 ghost layer, wall column", flush=True)
+    r = subprocess.run([DRIVER, "gri30", str(tmp_path / "in.bin"), str(tmp_path / "out.bin"), net],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ierr, sc, hr, cs, st = _read_out(tmp_path / "out.bin", SC_)
+    assert ierr == 0 and st[0] == 4 * 4 * 3 - 3 and st[10] == 0
+    fc = FiniteChem("gri30", 1e-6, 1e-12)
+    fc.load_surrogate(net)
+    py = SC_.copy(order="F")
+    hr_py = np.zeros(SC_.shape[:3], order="F")
+    _, ss = fc.source_chemistry_surrogate(RHO_, py, g, dt, P, P, mask=mask, HR_gp=hr_py)
+    np.testing.assert_array_equal(sc, py)
+    np.testing.assert_array_equal(hr, hr_py)
+    plain = SC_.copy(order="F")
+    FiniteChem("gri30", 1e-6, 1e-12).source_chemistry(RHO_, plain, g, dt, P, P, mask=mask)
+    nsp = g.isc_T - g.isc_sc
+    isc0 = g.isc_sc - 1
+    assert np.abs(sc[..., isc0:isc0 + nsp] - plain[..., isc0:isc0 + nsp]).max() <= 1e-6
+    print("accepted %d of %d cells" % (ss["accepted"], st[0]))

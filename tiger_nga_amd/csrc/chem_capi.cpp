@@ -90,6 +90,7 @@ struct tgr_chem {
   // surrogate-guided step: network (borrowed), feature / output scaling,
   // network inputs and outputs, rejected-cell list and counters
   tgr_mlp* net = nullptr;
+  tgr_mlp* own_net = nullptr;  // a network loaded by tgr_chem_load_surrogate (destroyed with the context)
   int net_in_pad = 0, net_out_pad = 0;
   double* d_elem = nullptr;  // [TGR_NELEM][64] element atoms / W (surrogate projection)
   float* d_in_shift = nullptr;
@@ -330,6 +331,7 @@ int tgr_chem_destroy(tgr_chem* c) {
   for (void* q : {(void*)c->d_in_shift, (void*)c->d_in_scale, (void*)c->d_out_scale, (void*)c->d_rej_count,
                   (void*)c->d_stotals})
     if (q) (void)hipFree(q);
+  if (c->own_net) tgr_mlp_destroy(c->own_net);
   c->b_x.release();
   c->b_guess.release();
   c->b_rej.release();
@@ -477,27 +479,27 @@ int tgr_chem_surrogate_features_device(tgr_chem* c, double dt, const tgr_grid* g
   return 0;
 }
 
-int tgr_chem_source_chemistry_surrogate_device(tgr_chem* c, double dt, double p_thermo, double p_thermo_old,
-                                               const tgr_grid* g, double* RHO_, double* SC_, const int* mask,
-                                               double* HR_gp, double* chemsrc_gp, void* stream, tgr_stats* stats,
-                                               tgr_surrogate_stats* sstats) {
-  if (!c || !RHO_ || !SC_) return fail(TGR_E_ARG, "NULL argument");
+}  // extern "C"
+
+namespace {
+
+// The surrogate-guided step on device fields whose species block starts at
+// scalar plane isc_sc0 and whose T is plane isc_T0 (0-based).
+int run_surrogate(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const tgr_grid* g, double* RHO_,
+                  double* SC_, int isc_sc0, int isc_T0, const int* mask, double* HR_gp, double* chemsrc_gp,
+                  hipStream_t s, tgr_stats* stats, tgr_surrogate_stats* sstats) {
+  int rc = 0;
   if (!c->net) return fail(TGR_E_ARG, "no network set (tgr_chem_set_surrogate)");
-  int rc = check_grid(g, c->nsp);
-  if (rc) return rc;
   if (!(dt > 0.0) || !std::isfinite(dt)) return fail(TGR_E_ARG, "dt must be positive and finite");
   if (stats) memset(stats, 0, sizeof(*stats));
   if (sstats) memset(sstats, 0, sizeof(*sstats));
-  hipStream_t s = (hipStream_t)stream;
-  StepParams p = step_params(c, dt, p_thermo, p_thermo_old, g, RHO_, SC_, g->isc_sc - 1, g->isc_T - 1, mask, HR_gp,
-                             chemsrc_gp);
+  StepParams p = step_params(c, dt, p_thermo, p_thermo_old, g, RHO_, SC_, isc_sc0, isc_T0, mask, HR_gp, chemsrc_gp);
   if (p.nx <= 0 || p.ny <= 0 || p.nz <= 0) return 0;
   if (c->surr_probe > 0 && c->surr_last_accept < c->surr_min_accept && c->surr_skipped + 1 < c->surr_probe) {
     // the guided solve has not been paying: plain step only
     c->surr_skipped++;
     if (sstats) sstats->guided = 0;
-    rc = run_step(c, dt, p_thermo, p_thermo_old, g, RHO_, SC_, g->isc_sc - 1, g->isc_T - 1, mask, HR_gp, chemsrc_gp,
-                  s, stats);
+    rc = run_step(c, dt, p_thermo, p_thermo_old, g, RHO_, SC_, isc_sc0, isc_T0, mask, HR_gp, chemsrc_gp, s, stats);
     if (!rc && sstats && stats) {
       sstats->rejected = stats->cells;
       sstats->fallback_ms = stats->kernel_ms;
@@ -572,8 +574,8 @@ int tgr_chem_source_chemistry_surrogate_device(tgr_chem* c, double dt, double p_
   if (tgr_launch_surr(&P, c->neq, grid_s, s)) return fail(TGR_E_HIP, "surrogate kernel launch failed");
   if (timing) HIP_TRY(hipEventRecord(c->ev2, s));
   // the plain integrator on the rejected cells
-  if ((rc = run_step(c, dt, p_thermo, p_thermo_old, g, RHO_, SC_, g->isc_sc - 1, g->isc_T - 1, mask, HR_gp,
-                     chemsrc_gp, s, nullptr, P.rej_list, c->d_rej_count)))
+  if ((rc = run_step(c, dt, p_thermo, p_thermo_old, g, RHO_, SC_, isc_sc0, isc_T0, mask, HR_gp, chemsrc_gp, s,
+                     nullptr, P.rej_list, c->d_rej_count)))
     return rc;
   // the accepted fraction steers the adaptive use: read back after every
   // guided call (a small copy; the call is then synchronous)
@@ -621,9 +623,12 @@ int tgr_chem_source_chemistry_surrogate_device(tgr_chem* c, double dt, double p_
   return 0;
 }
 
-int tgr_chem_source_chemistry(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const tgr_grid* g,
-                              double* RHO_, double* SC_, const int* mask, double* HR_gp, double* chemsrc_gp,
-                              tgr_stats* stats) {
+// host arrays: stage the block's density, species and T (and mask,
+// diagnostics) in device buffers, run the plain or the surrogate-guided step,
+// copy the results back; synchronous
+int host_step(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const tgr_grid* g, double* RHO_,
+              double* SC_, const int* mask, double* HR_gp, double* chemsrc_gp, tgr_stats* stats, bool surrogate,
+              tgr_surrogate_stats* sstats) {
   if (!c || !RHO_ || !SC_) return fail(TGR_E_ARG, "NULL argument");
   int rc = check_grid(g, c->nsp);
   if (rc) return rc;
@@ -646,10 +651,13 @@ int tgr_chem_source_chemistry(tgr_chem* c, double dt, double p_thermo, double p_
   if (mask) HIP_TRY(hipMemcpyAsync(c->b_mask.p, mask, mbytes, hipMemcpyHostToDevice, s));
   if (HR_gp) HIP_TRY(hipMemcpyAsync(c->b_hr.p, HR_gp, pb, hipMemcpyHostToDevice, s));
   if (chemsrc_gp) HIP_TRY(hipMemcpyAsync(c->b_chem.p, chemsrc_gp, 3 * pb, hipMemcpyHostToDevice, s));
+  const int* dm = mask ? (const int*)c->b_mask.p : nullptr;
+  double* dh = HR_gp ? (double*)c->b_hr.p : nullptr;
+  double* dc = chemsrc_gp ? (double*)c->b_chem.p : nullptr;
   tgr_stats local;
-  rc = run_step(c, dt, p_thermo, p_thermo_old, g, d_rho, d_sc, 0, nsp, mask ? (const int*)c->b_mask.p : nullptr,
-                HR_gp ? (double*)c->b_hr.p : nullptr, chemsrc_gp ? (double*)c->b_chem.p : nullptr, s,
-                stats ? stats : &local);
+  if (surrogate) rc = run_surrogate(c, dt, p_thermo, p_thermo_old, g, d_rho, d_sc, 0, nsp, dm, dh, dc, s,
+                                    stats ? stats : &local, sstats);
+  else rc = run_step(c, dt, p_thermo, p_thermo_old, g, d_rho, d_sc, 0, nsp, dm, dh, dc, s, stats ? stats : &local);
   if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(SC_ + (size_t)(g->isc_sc - 1) * plane, d_sc, pb * nsp, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(SC_ + (size_t)(g->isc_T - 1) * plane, d_sc + (size_t)nsp * plane, pb,
@@ -658,6 +666,81 @@ int tgr_chem_source_chemistry(tgr_chem* c, double dt, double p_thermo, double p_
   if (chemsrc_gp) HIP_TRY(hipMemcpyAsync(chemsrc_gp, c->b_chem.p, 3 * pb, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tgr_chem_source_chemistry_surrogate_device(tgr_chem* c, double dt, double p_thermo, double p_thermo_old,
+                                               const tgr_grid* g, double* RHO_, double* SC_, const int* mask,
+                                               double* HR_gp, double* chemsrc_gp, void* stream, tgr_stats* stats,
+                                               tgr_surrogate_stats* sstats) {
+  if (!c || !RHO_ || !SC_) return fail(TGR_E_ARG, "NULL argument");
+  int rc = check_grid(g, c->nsp);
+  if (rc) return rc;
+  return run_surrogate(c, dt, p_thermo, p_thermo_old, g, RHO_, SC_, g->isc_sc - 1, g->isc_T - 1, mask, HR_gp,
+                       chemsrc_gp, (hipStream_t)stream, stats, sstats);
+}
+
+int tgr_chem_source_chemistry_surrogate(tgr_chem* c, double dt, double p_thermo, double p_thermo_old,
+                                        const tgr_grid* g, double* RHO_, double* SC_, const int* mask, double* HR_gp,
+                                        double* chemsrc_gp, tgr_stats* stats, tgr_surrogate_stats* sstats) {
+  return host_step(c, dt, p_thermo, p_thermo_old, g, RHO_, SC_, mask, HR_gp, chemsrc_gp, stats, true, sstats);
+}
+
+int tgr_chem_load_surrogate(tgr_chem* c, const char* path) {
+  if (!c || !path) return fail(TGR_E_ARG, "NULL argument");
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(TGR_E_ARG, std::string("cannot open ") + path);
+  std::vector<char> raw;
+  {
+    char buf[65536];
+    size_t n;
+    while ((n = fread(buf, 1, sizeof(buf), f)) > 0) raw.insert(raw.end(), buf, buf + n);
+    fclose(f);
+  }
+  if (raw.size() < 32 || memcmp(raw.data(), "TGRMLP01", 8) != 0)
+    return fail(TGR_E_ARG, std::string(path) + ": not a network file");
+  int32_t shp[4];
+  int64_t nparam = 0;
+  memcpy(shp, raw.data() + 8, sizeof(shp));
+  memcpy(&nparam, raw.data() + 24, sizeof(nparam));
+  const int nin = shp[0], nout = shp[1];
+  if (nin != c->nsp + 2 || nout != c->neq)
+    return fail(TGR_E_ARG, std::string(path) + ": network shape does not match the mechanism");
+  if (nparam <= 0 || raw.size() != (size_t)32 + 8 * (size_t)nin + 8 * (size_t)nout + 4 * (size_t)nparam)
+    return fail(TGR_E_ARG, std::string(path) + ": size does not match its header");
+  std::vector<float> shift(nin), scale(nin), params((size_t)nparam);
+  std::vector<double> oscale(nout);
+  size_t o = 32;
+  memcpy(shift.data(), raw.data() + o, 4 * (size_t)nin);
+  memcpy(scale.data(), raw.data() + o + 4 * (size_t)nin, 4 * (size_t)nin);
+  o += 8 * (size_t)nin;
+  memcpy(oscale.data(), raw.data() + o, 8 * (size_t)nout);
+  o += 8 * (size_t)nout;
+  memcpy(params.data(), raw.data() + o, 4 * (size_t)nparam);
+  tgr_mlp* net = nullptr;
+  if (tgr_mlp_create(nin, nout, shp[2], shp[3], &net)) return fail(TGR_E_ARG, tgr_mlp_last_error());
+  int64_t np = 0;
+  int rc = 0;
+  if (tgr_mlp_layout(net, &np, nullptr, nullptr) || np != nparam)
+    rc = fail(TGR_E_ARG, std::string(path) + ": parameter count does not match the network shape");
+  if (!rc && tgr_mlp_set_params(net, params.data())) rc = fail(TGR_E_ARG, tgr_mlp_last_error());
+  if (!rc) rc = tgr_chem_set_surrogate(c, net, shift.data(), scale.data(), oscale.data());
+  if (rc) {
+    tgr_mlp_destroy(net);
+    return rc;
+  }
+  if (c->own_net) tgr_mlp_destroy(c->own_net);
+  c->own_net = net;
+  return 0;
+}
+
+int tgr_chem_source_chemistry(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const tgr_grid* g,
+                              double* RHO_, double* SC_, const int* mask, double* HR_gp, double* chemsrc_gp,
+                              tgr_stats* stats) {
+  return host_step(c, dt, p_thermo, p_thermo_old, g, RHO_, SC_, mask, HR_gp, chemsrc_gp, stats, false, nullptr);
 }
 
 int tgr_chem_prodrates_device(tgr_chem* c, int64_t n, const double* C, const double* T, const double* P,

@@ -146,6 +146,9 @@ def load_library(path: str = LIB_PATH):
                                                        ctypes.c_int64, vp]
     lib.tgr_chem_source_chemistry_surrogate_device.argtypes = step_args + [vp, ctypes.POINTER(_Stats),
                                                                            ctypes.POINTER(_SurrStats)]
+    lib.tgr_chem_source_chemistry_surrogate.argtypes = step_args + [ctypes.POINTER(_Stats),
+                                                                    ctypes.POINTER(_SurrStats)]
+    lib.tgr_chem_load_surrogate.argtypes = [vp, ctypes.c_char_p]
     lib.tgr_chem_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -235,6 +238,11 @@ class FiniteChem:
             self._h, net._h, fs.ctypes.data_as(fptr), fc.ctypes.data_as(fptr),
             oc.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "set_surrogate")
 
+    def load_surrogate(self, path: str):
+        """Load and attach a network file (the context owns it)."""
+        self._net = None
+        _check(self.lib, self.lib.tgr_chem_load_surrogate(self._h, path.encode()), "load_surrogate")
+
     def surrogate_features(self, SC_, grid: Grid, dt: float, X, stream=None):
         """Network inputs of the block's interior cells into X (bfloat16
         [rows, in_pad], rows a multiple of 64 >= cells)."""
@@ -245,16 +253,25 @@ class FiniteChem:
     def source_chemistry_surrogate(self, RHO_, SC_, grid: Grid, dt: float, p_thermo: float,
                                    p_thermo_old: float | None = None, mask=None, HR_gp=None, chemsrc_gp=None,
                                    stream=None, sync: bool = True):
-        """The surrogate-guided step on device tensors; returns (step stats,
-        guided-solve stats) when sync, else None."""
+        """The surrogate-guided step on device tensors or host numpy arrays;
+        returns (step stats, guided-solve stats) when sync (always for host
+        arrays), else None."""
         if p_thermo_old is None:
             p_thermo_old = p_thermo
-        self._check_arrays(RHO_, SC_, grid, mask, HR_gp, chemsrc_gp, True)
+        dev = _is_device(SC_)
+        self._check_arrays(RHO_, SC_, grid, mask, HR_gp, chemsrc_gp, dev)
         st, ss = _Stats(), _SurrStats()
         g = grid._c()
-        rc = self.lib.tgr_chem_source_chemistry_surrogate_device(
-            self._h, dt, p_thermo, p_thermo_old, ctypes.byref(g), _ptr(RHO_), _ptr(SC_), _ptr(mask), _ptr(HR_gp),
-            _ptr(chemsrc_gp), stream, ctypes.byref(st) if sync else None, ctypes.byref(ss) if sync else None)
+        if dev:
+            rc = self.lib.tgr_chem_source_chemistry_surrogate_device(
+                self._h, dt, p_thermo, p_thermo_old, ctypes.byref(g), _ptr(RHO_), _ptr(SC_), _ptr(mask),
+                _ptr(HR_gp), _ptr(chemsrc_gp), stream, ctypes.byref(st) if sync else None,
+                ctypes.byref(ss) if sync else None)
+        else:
+            sync = True
+            rc = self.lib.tgr_chem_source_chemistry_surrogate(
+                self._h, dt, p_thermo, p_thermo_old, ctypes.byref(g), _ptr(RHO_), _ptr(SC_), _ptr(mask),
+                _ptr(HR_gp), _ptr(chemsrc_gp), ctypes.byref(st), ctypes.byref(ss))
         _check(self.lib, rc, "source_chemistry_surrogate")
         if not sync:
             return None

@@ -33,6 +33,8 @@ module tiger_chem
   public :: tgr_chem_create, tgr_chem_destroy, tgr_chem_source_chemistry, tgr_chem_source_chemistry_device
   public :: tgr_chem_molar_mass, tgr_chem_species_name, tgr_chem_set_option, tgr_chem_last_error
   public :: tgr_chem_properties_device, tgr_chem_temperature_device
+  public :: tgr_chem_load_surrogate, tgr_chem_source_chemistry_surrogate
+  public :: tiger_chem_load_surrogate, tiger_chem_source_chemistry_surrogate
   public :: tiger_chem_init, tiger_chem_final, tiger_chem_source_chemistry, tiger_chem_message
   public :: tiger_chem_ctx
 
@@ -112,6 +114,23 @@ module tiger_chem
        type(c_ptr), value :: Y, hmix, T0, T, iters, stream
      end function tgr_chem_temperature_device
 
+     ! surrogate-guided step: load a trained network file, then the step on
+     ! host arrays (sstats: tgr_surrogate_stats or c_null_ptr)
+     integer(c_int) function tgr_chem_load_surrogate(ctx, path) bind(C, name="tgr_chem_load_surrogate")
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: ctx
+       character(kind=c_char), dimension(*), intent(in) :: path
+     end function tgr_chem_load_surrogate
+
+     integer(c_int) function tgr_chem_source_chemistry_surrogate(ctx, dt, p_thermo, p_thermo_old, g, rho, sc, &
+          mask, hr, chemsrc, stats, sstats) bind(C, name="tgr_chem_source_chemistry_surrogate")
+       import :: c_int, c_ptr, c_double, tgr_grid
+       type(c_ptr), value :: ctx
+       real(c_double), value :: dt, p_thermo, p_thermo_old
+       type(tgr_grid), intent(in) :: g
+       type(c_ptr), value :: rho, sc, mask, hr, chemsrc, stats, sstats
+     end function tgr_chem_source_chemistry_surrogate
+
      type(c_ptr) function tgr_chem_last_error() bind(C, name="tgr_chem_last_error")
        import :: c_ptr
      end function tgr_chem_last_error
@@ -186,5 +205,44 @@ contains
     ierr = tgr_chem_source_chemistry(tiger_chem_ctx, dt, P_thermo, P_thermo_old, g, c_loc(RHO_), c_loc(SC_), &
          pm, ph, pc, ps)
   end subroutine tiger_chem_source_chemistry
+
+  ! attach a trained network file (e.g. tiger_nga_amd/models/gri30.bin)
+  subroutine tiger_chem_load_surrogate(path, ierr)
+    character(len=*), intent(in) :: path
+    integer, intent(out) :: ierr
+    ierr = tgr_chem_load_surrogate(tiger_chem_ctx, trim(path) // c_null_char)
+  end subroutine tiger_chem_load_surrogate
+
+  ! tiger_chem_source_chemistry by the surrogate-guided path: cells the
+  ! guided solve does not accept are integrated by the plain step in the same
+  ! call
+  subroutine tiger_chem_source_chemistry_surrogate(RHO_, SC_, g, dt, P_thermo, P_thermo_old, ierr, mask, HR_gp, &
+       chemsrc_gp, stats)
+    type(tgr_grid), intent(in) :: g
+    real(c_double), intent(inout), target, contiguous :: RHO_(:,:,:)
+    real(c_double), intent(inout), target, contiguous :: SC_(:,:,:,:)
+    real(c_double), intent(in) :: dt, P_thermo, P_thermo_old
+    integer, intent(out) :: ierr
+    integer(c_int), intent(in), target, contiguous, optional :: mask(:,:)
+    real(c_double), intent(inout), target, contiguous, optional :: HR_gp(:,:,:)
+    real(c_double), intent(inout), target, contiguous, optional :: chemsrc_gp(:,:,:,:)
+    type(tgr_stats), intent(out), target, optional :: stats
+    type(c_ptr) :: pm, ph, pc, ps
+    pm = c_null_ptr
+    ph = c_null_ptr
+    pc = c_null_ptr
+    ps = c_null_ptr
+    if (present(mask)) pm = c_loc(mask)
+    if (present(HR_gp)) ph = c_loc(HR_gp)
+    if (present(chemsrc_gp)) pc = c_loc(chemsrc_gp)
+    if (present(stats)) ps = c_loc(stats)
+    if (size(RHO_, 1) /= g%imaxo_ - g%imino_ + 1 .or. size(RHO_, 2) /= g%jmaxo_ - g%jmino_ + 1 .or. &
+        size(RHO_, 3) /= g%kmaxo_ - g%kmino_ + 1 .or. size(SC_, 4) /= g%nscalar) then
+       ierr = TGR_E_ARG
+       return
+    end if
+    ierr = tgr_chem_source_chemistry_surrogate(tiger_chem_ctx, dt, P_thermo, P_thermo_old, g, c_loc(RHO_), &
+         c_loc(SC_), pm, ph, pc, ps, c_null_ptr)
+  end subroutine tiger_chem_source_chemistry_surrogate
 
 end module tiger_chem

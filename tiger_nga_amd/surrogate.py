@@ -10,12 +10,14 @@ the surrogate step (tgr_chem_surrogate_features_device), so training and use
 see identical features.  Targets are increments divided by out_scale, the
 per-component RMS increment of the training set.
 
-Model file (.npz, numbers only): params (FP32 flat vector of
-include/tiger_mlp.h), in_shift, in_scale, out_scale, shape
-[nin, nout, hidden, layers], and a JSON string ``meta``.
+Model file (``.bin``, little-endian numbers only, read by the C ABI's
+tgr_chem_load_surrogate as well): the 8 bytes ``TGRMLP01``; int32 nin, nout,
+hidden, layers; int64 nparam; float32 in_shift[nin], in_scale[nin]; float64
+out_scale[nout]; float32 params[nparam] (the flat vector of
+include/tiger_mlp.h).  The training report sits beside it as ``.json``.
 
 Usage (on a GPU): ``python -m tiger_nga_amd.surrogate --mech gri30 --out
-tiger_nga_amd/models/gri30.npz``.
+tiger_nga_amd/models/gri30.bin``.
 """
 from __future__ import annotations
 
@@ -51,8 +53,11 @@ TRAIN_SETS = {
 }
 
 
+MAGIC = b"TGRMLP01"
+
+
 def model_path(mech: str) -> str:
-    return os.path.join(MODELS, "%s.npz" % mech)
+    return os.path.join(MODELS, "%s.bin" % mech)
 
 
 def input_scaling(mech: str, Y, T, dt_ref: float):
@@ -184,18 +189,47 @@ class Surrogate:
 
     def save(self, path):
         n = self.net
-        np.savez_compressed(path, params=n.get_params(), in_shift=self.in_shift, in_scale=self.in_scale,
-                            out_scale=self.out_scale, shape=np.array([n.nin, n.nout, n.hidden, n.layers]),
-                            meta=np.array(json.dumps(self.meta)))
+        params = n.get_params()
+        with open(path, "wb") as f:
+            f.write(MAGIC)
+            f.write(np.array([n.nin, n.nout, n.hidden, n.layers], dtype="<i4").tobytes())
+            f.write(np.array([params.size], dtype="<i8").tobytes())
+            f.write(np.asarray(self.in_shift, dtype="<f4").tobytes())
+            f.write(np.asarray(self.in_scale, dtype="<f4").tobytes())
+            f.write(np.asarray(self.out_scale, dtype="<f8").tobytes())
+            f.write(params.astype("<f4").tobytes())
+        with open(os.path.splitext(path)[0] + ".json", "w") as f:
+            json.dump(self.meta, f, indent=1)
+
+    @staticmethod
+    def read(path):
+        """(shape, in_shift, in_scale, out_scale, params, meta) of a model file."""
+        with open(path, "rb") as f:
+            raw = f.read()
+        if raw[:8] != MAGIC:
+            raise ValueError("%s: not a network file" % path)
+        nin, nout, hidden, layers = (int(v) for v in np.frombuffer(raw, "<i4", 4, 8))
+        nparam = int(np.frombuffer(raw, "<i8", 1, 24)[0])
+        o = 32
+        in_shift = np.frombuffer(raw, "<f4", nin, o)
+        in_scale = np.frombuffer(raw, "<f4", nin, o + 4 * nin)
+        o += 8 * nin
+        out_scale = np.frombuffer(raw, "<f8", nout, o)
+        o += 8 * nout
+        params = np.frombuffer(raw, "<f4", nparam, o)
+        if o + 4 * nparam != len(raw):
+            raise ValueError("%s: size does not match its header" % path)
+        mp = os.path.splitext(path)[0] + ".json"
+        meta_info = json.load(open(mp)) if os.path.exists(mp) else {}
+        return (nin, nout, hidden, layers), in_shift.copy(), in_scale.copy(), out_scale.copy(), params.copy(), meta_info
 
     @staticmethod
     def load(path):
         from .mlp import MLP
-        z = np.load(path, allow_pickle=False)
-        nin, nout, hidden, layers = (int(v) for v in z["shape"])
+        (nin, nout, hidden, layers), in_shift, in_scale, out_scale, params, meta_info = Surrogate.read(path)
         net = MLP(nin, nout, hidden, layers)
-        net.set_params(z["params"])
-        return Surrogate(net, z["in_shift"], z["in_scale"], z["out_scale"], json.loads(str(z["meta"])))
+        net.set_params(params)
+        return Surrogate(net, in_shift, in_scale, out_scale, meta_info)
 
 
 def build(mech: str, seeds=range(1, 9), steps: int = 20000, hidden: int = 256, layers: int = 4, log=print):
