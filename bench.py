@@ -145,6 +145,7 @@ def bench_surrogate(a, torch, fc, g, R_d, S0_d, S_d, HR_d, CS_d, dt, P, mech, sp
         dT = (S_d[..., nsp] - ref[..., nsp]).abs()
         return float(dY.max()), float(dY.pow(2).mean().sqrt()), float(dT.max())
 
+    from tiger_nga_amd import flops
     fc.set_option("surr_probe", 0)
     t_forced = _time_steps(torch, a, guided, S_d, S0_d)
     S_d.copy_(S0_d)
@@ -172,8 +173,13 @@ def bench_surrogate(a, torch, fc, g, R_d, S0_d, S_d, HR_d, CS_d, dt, P, mech, sp
     r_max, r_rms, rT = errors(Rp)
     fc.set_option("surr_probe", 8)
     fc.set_surrogate(None, None, None, None)
+    gf = flops.guided_flops(fc.info, ss, int(fc.options.get("surr_nsub", 4))) if ss["surrogate_ms"] > 0 else 0.0
     return {
         "_state": state,
+        "guided_roofline": {"bound": "mfma", "compute_unit": "FP64 vector ALU", "achieved": gf / (
+            ss["surrogate_ms"] * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+            "frac": gf / (ss["surrogate_ms"] * 1e-3) / PEAK_FP64, "flops": gf,
+            "note": "features + network + surr_kernel time; flops of the FP64 guided solve only"},
         "value": cells / t_adapt, "ms_per_step": t_adapt * 1e3, "model": os.path.relpath(path, ROOT),
         "policy": "adaptive: guided solve while its accepted fraction >= 0.15, else plain step with a guided "
                   "probe every 8 calls",

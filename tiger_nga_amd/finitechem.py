@@ -182,6 +182,7 @@ class FiniteChem:
     def __init__(self, mech: str, rtol: float = 1e-6, atol: float = 1e-12, max_steps: int = 10000):
         self.lib = load_library()
         self.mech = mech
+        self.options = {}  # knobs set through set_option
         self.rtol, self.atol = rtol, atol
         h = ctypes.c_void_p()
         _check(self.lib, self.lib.tgr_chem_create(mech.encode(), rtol, atol, max_steps, ctypes.byref(h)),
@@ -222,6 +223,7 @@ class FiniteChem:
             _check(self.lib, self.lib.tgr_chem_set_option_real(self._h, key.encode(), float(value)), "set_option")
         else:
             _check(self.lib, self.lib.tgr_chem_set_option(self._h, key.encode(), int(value)), "set_option")
+        self.options[key] = value
 
     def set_surrogate(self, net, in_shift, in_scale, out_scale):
         """Attach a trained tiger_nga_amd.mlp.MLP (None detaches) with its

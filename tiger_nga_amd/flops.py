@@ -66,3 +66,18 @@ def properties_flops(nsp: int) -> float:
     weighted term and its accumulation), conductivity and the two mixture sums
     (12 per species), density and outputs (5 + nsp)."""
     return 3 * nsp + 19 * nsp + 10 * nsp * nsp + 12 * nsp + 5 + nsp
+
+
+def guided_flops(info: dict, sstats: dict, nsub: int) -> float:
+    """Algorithmic flops of the surrogate step's guided solve (before the plain
+    fallback) from its counters: every right-hand side (Newton and
+    difference-quotient columns), one LU of I - gamma h J and one filtered
+    error estimate per sub-step of every attempted cell, a triangular solve
+    and update per Newton iteration, and the enthalpy check / element
+    projection (taken as 60 neq per cell)."""
+    n = info["nsp"] + 1
+    cells = sstats["accepted"] + sstats["rejected"]
+    f_lu = 2.0 * n * n + 2.0 / 3.0 * n ** 3
+    f_solve = 2.0 * n * n + 4.0 * n
+    return (sstats["rhs_evals"] * rhs_flops(info) + cells * nsub * (f_lu + f_solve)
+            + sstats["newton_iters"] * f_solve + cells * 60.0 * n)
