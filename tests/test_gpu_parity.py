@@ -213,3 +213,40 @@ def test_step_graph_replay(chem, torch_dev):
         assert torch.equal(HR, ref_hr), "replay %d heat release differs" % r
         _ = (S_d * 2).sum().item()  # eager work between replays
     print("graph replay: 3 replays identical to the eager step")
+
+
+def test_step_failure_paths(chem, oracle, torch_dev):
+    """Failure handling through the C ABI (the reference ignores CVODE's
+    return code, finitechem.f90:1034): with MAX_NSTEPS = 5 every cell stops
+    with the state of its last accepted step, as in the oracle, and is counted
+    as failed; a NaN and a negative temperature fail their own cells only
+    (the NaN cell stays NaN), the other cells match the oracle.  The same
+    cases run first in the CPU wave emulator (tests/test_wave_emu.py), which
+    shows that every wave ends."""
+    from tiger_nga_amd.finitechem import FiniteChem, Grid
+    _, Y, T, rho, P, dt = inputs.states("gri30", n=8, seed=8)
+    RHO_, SC_, isc_sc, isc_T = inputs.fortran_fields(Y, T, rho, 2, 2, 2)
+    g = Grid.block(2, 2, 2, SC_.shape[3], isc_sc, isc_T)
+    fc = FiniteChem("gri30", 1e-6, 1e-12, max_steps=5)
+    S = SC_.copy(order="F")
+    st = fc.source_chemistry(RHO_, S, g, dt, P, P)
+    assert st["failed"] == 8 and st["nst"] == 8 * 5
+    ref = oracle.integrate_batch("gri30", inputs.packed_state(Y, T, rho), dt, P, rtol=1e-6, atol=1e-12,
+                                 max_steps=5)
+    nsp = isc_T - isc_sc
+    got = np.stack([S[..., k].ravel(order="F") for k in range(nsp)], axis=1)
+    assert np.abs(rho[:, None] * got - ref[:, :nsp]).max() / np.abs(ref[:, :nsp]).max() < 1e-9
+    # NaN / negative temperature among good cells
+    _, Y, T, rho, P, dt = inputs.states("h2", n=4, seed=9)
+    T = T.copy()
+    T[1], T[2] = np.nan, -300.0
+    RHO_, SC_, isc_sc, isc_T = inputs.fortran_fields(Y, T, rho, 4, 1, 1)
+    g = Grid.block(4, 1, 1, SC_.shape[3], isc_sc, isc_T)
+    S = SC_.copy(order="F")
+    st = chem("h2").source_chemistry(RHO_, S, g, dt, P, P)
+    assert st["failed"] == 2
+    assert np.isnan(S[1, 0, 0, isc_T - 1])
+    ref = oracle.integrate_batch("h2", inputs.packed_state(Y, T, rho), dt, P, rtol=1e-6, atol=1e-12)
+    for c in (0, 3):
+        assert abs(S[c, 0, 0, isc_T - 1] - ref[c, -1] / rho[c]) < 1e-6
+    print("\nfailure paths: MAX_NSTEPS=5 -> 8/8 failed with the oracle's states; NaN / negative T -> 2 failed")

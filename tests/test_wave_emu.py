@@ -28,7 +28,8 @@ def emu():
 
 
 @pytest.mark.parametrize("workload,n,rtol,atol", [("h2", 6, 1e-6, 1e-12), ("h2", 2, 1e-10, 1e-16),
-                                                  ("gri30", 2, 1e-6, 1e-12), ("dme39", 1, 1e-6, 1e-12)])
+                                                  ("gri30", 2, 1e-6, 1e-12), ("dme39", 1, 1e-6, 1e-12),
+                                                  ("ethyred", 1, 1e-6, 1e-12)])
 def test_emulated_kernel_matches_oracle(emu, oracle, workload, n, rtol, atol):
     mech, Y, T, rho, P, dt = inputs.states(workload, n=n, seed=5)
     sc = np.ascontiguousarray(np.concatenate([Y.T, T[None, :]], axis=0))
@@ -51,7 +52,7 @@ def test_emulated_kernel_matches_oracle(emu, oracle, workload, n, rtol, atol):
     assert dT[same].max(initial=0.0) < 1e-6 and dT.max() < 1e-2
 
 
-@pytest.mark.parametrize("workload", ["h2", "gri30", "dme39"])
+@pytest.mark.parametrize("workload", ["h2", "gri30", "dme39", "ethyred"])
 def test_generated_lane_rhs_matches_oracle(emu, oracle, workload):
     """The generated lane-per-state right-hand side (lane_rhs_<mech>.h) against
     the oracle's table-driven one, error relative to the gross rates."""
@@ -71,7 +72,49 @@ def test_generated_lane_rhs_matches_oracle(emu, oracle, workload):
 
 def oracle_molar_mass(oracle, mech):
     import ctypes
-    n = {"h2": 9, "gri30": 36, "dme39": 39}[mech]
+    n = {"h2": 9, "gri30": 36, "dme39": 39, "ethyred": 45}[mech]
     W = np.zeros(n)
     oracle.lib().orc_molar_mass(oracle.MECH_ID[mech], W.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     return W
+
+
+def _emu_run(emu, mech, Y, T, rho, P, dt, rtol, atol, max_steps):
+    n = Y.shape[0]
+    sc = np.ascontiguousarray(np.concatenate([Y.T, T[None, :]], axis=0))
+    cnt = np.zeros((n, 10), dtype=np.int32)
+    hr, cs = np.zeros(n), np.zeros(3 * n)
+    rc = emu.emu_step(mech.encode(), n, rho.ctypes.data, sc.ctypes.data, dt, P, rtol, atol, max_steps,
+                      cnt.ctypes.data, hr.ctypes.data, cs.ctypes.data)
+    assert rc == 0
+    return sc, cnt
+
+
+def test_emulated_failure_paths_match_oracle(emu, oracle):
+    """Failure handling that the host can force (the reference ignores CVODE's
+    return code, finitechem.f90:1034): MAX_NSTEPS = 5 stops every cell with
+    too-much-work and the state of its last accepted step, as the oracle;
+    a NaN temperature is a bad-input cell left as it was; a negative
+    temperature ends in a failure status after a bounded number of steps;
+    the other cells of the same call are unaffected."""
+    mech, Y, T, rho, P, dt = inputs.states("gri30", n=3, seed=8)
+    sc, cnt = _emu_run(emu, mech, Y, T, rho, P, dt, 1e-6, 1e-12, 5)
+    ref, st = oracle.integrate_batch(mech, inputs.packed_state(Y, T, rho), dt, P, rtol=1e-6, atol=1e-12,
+                                     max_steps=5, with_stats=True)
+    assert (cnt[:, 9] == -1).all() and (st[:, 9] == -1).all() and (cnt[:, 0] == 5).all()
+    assert (cnt[:, :9] == st[:, :9]).all()
+    ew = (np.abs(rho[:, None] * sc[:-1].T - ref[:, :-1]) / (1e-6 * np.abs(ref[:, :-1]) + 1e-12)).max()
+    assert ew <= 0.1
+    # NaN and negative temperatures among good cells
+    mech, Y, T, rho, P, dt = inputs.states("h2", n=4, seed=9)
+    T = T.copy()
+    T[1] = np.nan
+    T[2] = -300.0
+    sc, cnt = _emu_run(emu, mech, Y, T, rho, P, dt, 1e-6, 1e-12, 10000)
+    ref, st = oracle.integrate_batch(mech, inputs.packed_state(Y, T, rho), dt, P, rtol=1e-6, atol=1e-12,
+                                     with_stats=True)
+    assert cnt[1, 9] == -6 and st[1, 9] == -6 and np.isnan(sc[-1, 1])
+    assert cnt[2, 9] < 0 and st[2, 9] < 0
+    assert cnt[0, 9] == 0 and cnt[3, 9] == 0
+    for c in (0, 3):
+        assert (cnt[c, :9] == st[c, :9]).all()
+        assert np.abs(sc[-1, c] - ref[c, -1] / rho[c]) < 1e-6
