@@ -11,7 +11,10 @@ Choices (all exposed as context options of include/tiger_chem.h):
 * ``surr_nsub`` / ``surr_grade``  sub-step schedule of the surrogate-guided
   solve (when a trained network exists), chosen on relaxed states (after one
   plain step, where the guided solve pays) for throughput among the schedules
-  whose max species error against the plain step stays <= 1e-6.
+  whose max species error against the plain step stays <= 1e-6;
+* ``surr_tol_y``  its acceptance threshold: the largest of 1e-6, 2e-6, 4e-6
+  whose measured max species error stays <= 5e-7 (half the 1e-6 bound) on
+  relaxed and on freshly generated states.
 
 Results are cached in ``tiger_nga_amd/tuning.json`` keyed by mechanism, device
 name and the sample size bucket; ``apply(fc)`` sets the cached choices on a
@@ -123,10 +126,36 @@ def tune(mech: str, cells: int = 32 ** 3, seed: int = 7, reps: int = 2, log=prin
             log("  surrogate nsub %d grade %.1f: %.2f ms, max |dY| %.2e" % (nsub, grade, t * 1e3, err))
             if err <= 1e-6 and (best is None or t < best[0]):
                 best = (t, nsub, grade)
-        fc.set_surrogate(None, None, None, None)
         rec["measured"]["surrogate"] = st_all
         if best:
             rec["surr_nsub"], rec["surr_grade"] = best[1], best[2]
+            fc.set_option("surr_nsub", best[1])
+            fc.set_option("surr_grade", best[2])
+            # acceptance threshold: the largest whose measured max species
+            # error stays <= 5e-7 on both the relaxed and the generated states
+            # (half the 1e-6 bound: larger blocks reach further into the tail)
+            R0, F0, g0, P0 = surrogate.block_states(fc, WORKLOAD[mech], cells, seed, 0)
+            fc.set_surrogate(None, None, None, None)
+            ref0 = F0.clone()
+            fc.source_chemistry(R0, ref0, g0, dt, P0, P0)
+            sur.attach(fc)
+            fc.set_option("surr_probe", 0)
+            tol_ok = 1e-6
+            for tol in (1e-6, 2e-6, 4e-6):
+                fc.set_option("surr_tol_y", tol)
+                errs = []
+                for (Rb, Sb0, gb, Pb, rb) in ((R_d, S0, g, Pg, ref), (R0, F0, g0, P0, ref0)):
+                    Sb = Sb0.clone()
+                    fc.source_chemistry_surrogate(Rb, Sb, gb, dt, Pb, Pb)
+                    errs.append(float((Sb[..., :nsp] - rb[..., :nsp]).abs().max()))
+                st_all["tol_y %.0e" % tol] = {"max_species_err_relaxed": errs[0], "max_species_err_generated": errs[1]}
+                log("  surrogate tol_y %.0e: max |dY| relaxed %.2e, generated %.2e" % (tol, errs[0], errs[1]))
+                if max(errs) <= 5e-7:
+                    tol_ok = tol
+                else:
+                    break
+            rec["surr_tol_y"] = tol_ok
+        fc.set_surrogate(None, None, None, None)
     fc.close()
     return rec
 
@@ -161,7 +190,7 @@ def apply(fc, cells: int, path=CACHE) -> dict:
     done = {}
     if not rec:
         return done
-    for k in ("rhs_mapping", "grid", "surr_nsub", "surr_grade"):
+    for k in ("rhs_mapping", "grid", "surr_nsub", "surr_grade", "surr_tol_y"):
         if k in rec:
             fc.set_option(k, rec[k])
             done[k] = rec[k]

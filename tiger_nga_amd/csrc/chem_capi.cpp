@@ -106,7 +106,7 @@ struct tgr_chem {
   // every surr_probe calls (0: always guided)
   int surr_probe = 8, surr_skipped = 0;
   double surr_min_accept = 0.15, surr_last_accept = 1.0;
-  double surr_newton_tol = 0.1, surr_tol_y = 1.0e-6, surr_tol_t = 1.0e-2, surr_grade = 1.0;
+  double surr_newton_tol = 0.1, surr_tol_y = 2.0e-6, surr_tol_t = 1.0e-2, surr_grade = 1.0;
   hipEvent_t ev2 = nullptr;
 };
 
