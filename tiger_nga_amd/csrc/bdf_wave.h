@@ -710,6 +710,132 @@ struct Bdf {
     return false;
   }
 
+  // The same method with step-size control: from h = h0 (or, with h0_fac > 0,
+  // h0_fac / max_k |f_k(y0)| wacc_k within [dt/1024, dt]), each step's error
+  // estimate e (as above) accepts the step when e <= 1 and scales the next
+  // step by min(4, max(0.2, 0.9 e^(-1/3))), a failed step is retried with the
+  // smaller h; the last step is cut to end at dt.  One Jacobian at the initial
+  // state, one LU per attempted step.  gat(t) is the network's prediction of
+  // the state at time t; each stage's Newton starts from the previous stage
+  // value moved by the prediction's change between the two stage times;
+  // without a network each stage starts from its known part plus gamma h
+  // times the previous derivative.  A stage whose Newton iteration does not
+  // converge (or a singular iteration matrix) retries the step at h / 4.
+  // Returns 0 (accepted), or after max_att attempted steps without reaching
+  // dt (or a non-finite error estimate) 1 if the last failed attempt was a
+  // Newton or singular-matrix failure, 2 if it failed the error test.
+  template <class GuessAt>
+  __device__ __forceinline__ int sdirk_adaptive(double& yv, double dt, double h0, double h0_fac, const GuessAt& gat,
+                                                bool use_guess, double wacc, int nmax, double ntol, int max_att,
+                                                int& nit) {
+    c_nfe = c_nfe_dq = c_nje = c_nsetups = c_nni = c_netf = c_ncfn = c_nlu = 0;
+    nst = 0;
+    const double y0 = act ? yv : 0.0;
+    set_ewt(y0);
+    const double f0 = f(y0);
+    c_nfe++;
+    if (h0_fac > 0.0) {
+      // first step from the initial rate: the step that moves the state by
+      // h0_fac acceptance tolerances at that rate
+      const double r0 = wave_max(act ? fabs(f0) * wacc : 0.0);
+      h0 = (r0 > 0.0) ? h0_fac / r0 : dt;
+      h0 = fmin(dt, fmax(dt * (1.0 / 1024.0), h0));
+    }
+    h = h0;
+    dq_jac(y0, f0);
+    c_nje++;
+    double yi = y0, t = 0.0, hh = fmin(h0, dt);
+    double kl = act ? f0 : 0.0;  // derivative at yi (the last stage's, the method being stiffly accurate)
+    int why = 2;  // reason of the last failed attempt: 1 Newton / singular, 2 error test
+    for (int att = 0; att < max_att; ++att) {
+      if (dt - t <= 1.0e-12 * dt) {
+        yv = yi;
+        return 0;
+      }
+      if (t + hh > dt || dt - (t + hh) < 1.0e-3 * hh) hh = dt - t;
+      const double gh = kSdG * hh;
+      {
+        const TGR_GLOBAL double* jr = fresh(jw) + lane;
+#pragma unroll
+        for (int c = 0; c < NEQ; ++c) {
+          double v = -gh * jr[c * kWave];
+          if (c == lane) v += 1.0;
+          M[c] = v;
+        }
+      }
+      c_nlu++;
+      if (getrf()) {
+        hh *= 0.25;  // singular iteration matrix: retry with a smaller step
+        why = 1;
+        continue;
+      }
+      const double rg = 1.0 / gh;
+      const double ts[3] = {t + kSdG * hh, t + kSdC2 * hh, t + hh};
+      // stage predictors: the network's change between the stage times, or
+      // without a network the previous derivative carried over the stage
+      double gp = use_guess ? gat(t) : 0.0;
+      double gc = 0.0;
+      double z = act ? yi + gh * kl : 0.0;
+      if (use_guess) {
+        gc = gat(ts[0]);
+        z = yi + (gc - gp);
+        gp = gc;
+      }
+      if (!sdirk_newton(z, yi, gh, wacc, nmax, ntol, nit)) {
+        hh *= 0.25;  // no convergence: retry with a smaller step
+        why = 1;
+        continue;
+      }
+      const double k1 = act ? (z - yi) * rg : 0.0;
+      const double base2 = act ? yi + kSdA21 * hh * k1 : 0.0;
+      if (use_guess) {
+        gc = gat(ts[1]);
+        z += gc - gp;
+        gp = gc;
+      } else {
+        z = act ? base2 + gh * k1 : 0.0;
+      }
+      if (!sdirk_newton(z, base2, gh, wacc, nmax, ntol, nit)) {
+        hh *= 0.25;
+        why = 1;
+        continue;
+      }
+      const double k2 = act ? (z - base2) * rg : 0.0;
+      const double base3 = act ? yi + hh * (kSdB1 * k1 + kSdB2 * k2) : 0.0;
+      if (use_guess) {
+        gc = gat(ts[2]);
+        z += gc - gp;
+      } else {
+        z = act ? base3 + gh * k2 : 0.0;
+      }
+      if (!sdirk_newton(z, base3, gh, wacc, nmax, ntol, nit)) {
+        hh *= 0.25;
+        why = 1;
+        continue;
+      }
+      const double k3 = act ? (z - base3) * rg : 0.0;
+      const double e =
+          getrs(act ? hh * ((kSdB1 - kSdBh1) * k1 + (kSdB2 - kSdBh2) * k2 + kSdG * k3) : 0.0);
+      const double en = wave_max(act ? fabs(e) * wacc : 0.0);
+      if (!isfinite(en)) return 1;
+      const double fac = fmin(4.0, fmax(0.2, 0.9 * rpow(fmax(en, 1.0e-10), -1.0 / 3.0)));
+      if (en <= 1.0) {
+        nst++;
+        t += hh;
+        yi = z;
+        kl = k3;
+      } else {
+        why = 2;
+      }
+      hh *= fac;
+    }
+    if (dt - t <= 1.0e-12 * dt) {
+      yv = yi;
+      return 0;
+    }
+    return why;
+  }
+
   template <class Guess>
   __device__ __forceinline__ int sdirk_guided(double& yv, int nsub, const double* hs, const Guess& guess,
                                               bool use_guess, double wacc, int nmax, double ntol, int& nit) {

@@ -105,6 +105,8 @@ struct tgr_chem {
   // below surr_min_accept, calls go to the plain step, with a guided probe
   // every surr_probe calls (0: always guided)
   int surr_probe = 8, surr_skipped = 0;
+  int surr_adaptive = 0, surr_max_att = 48;
+  double surr_h0_fac = 0.0;
   double surr_min_accept = 0.15, surr_last_accept = 1.0;
   double surr_newton_tol = 0.1, surr_tol_y = 2.0e-6, surr_tol_t = 1.0e-2, surr_grade = 1.0;
   hipEvent_t ev2 = nullptr;
@@ -381,6 +383,12 @@ int tgr_chem_set_option(tgr_chem* c, const char* key, int64_t v) {
   } else if (!strcmp(key, "surr_guess")) {
     if (v != 0 && v != 1) return fail(TGR_E_ARG, "surr_guess must be 0 or 1");
     c->surr_guess = (int)v;
+  } else if (!strcmp(key, "surr_adaptive")) {
+    if (v != 0 && v != 1) return fail(TGR_E_ARG, "surr_adaptive must be 0 or 1");
+    c->surr_adaptive = (int)v;
+  } else if (!strcmp(key, "surr_max_att")) {
+    if (v < 1 || v > 1000) return fail(TGR_E_ARG, "surr_max_att must be 1..1000");
+    c->surr_max_att = (int)v;
   } else if (!strcmp(key, "surr_probe")) {
     if (v < 0 || v > 1000000) return fail(TGR_E_ARG, "surr_probe must be >= 0");
     c->surr_probe = (int)v;
@@ -445,7 +453,13 @@ int tgr_chem_set_surrogate(tgr_chem* c, tgr_mlp* net, const float* in_shift, con
 
 int tgr_chem_set_option_real(tgr_chem* c, const char* key, double v) {
   if (!c || !key) return fail(TGR_E_ARG, "NULL argument");
-  if (!(v > 0.0) || !std::isfinite(v)) return fail(TGR_E_ARG, "value must be positive and finite");
+  if (!std::isfinite(v)) return fail(TGR_E_ARG, "value must be finite");
+  if (!strcmp(key, "surr_h0_fac")) {
+    if (v < 0.0) return fail(TGR_E_ARG, "surr_h0_fac must be >= 0 (0: first sub-step of the schedule)");
+    c->surr_h0_fac = v;
+    return 0;
+  }
+  if (!(v > 0.0)) return fail(TGR_E_ARG, "value must be positive and finite");
   if (!strcmp(key, "surr_tol_y")) c->surr_tol_y = v;
   else if (!strcmp(key, "surr_tol_t")) c->surr_tol_t = v;
   else if (!strcmp(key, "surr_newton_tol")) c->surr_newton_tol = v;
@@ -509,7 +523,9 @@ int run_surrogate(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, 
   c->surr_skipped = 0;
   if (sstats) sstats->guided = 1;
   const int64_t rows = (p.ncell + 63) / 64 * 64;
-  const int nst = tgr::kSurrStages * c->surr_nsub;
+  // network output times: every stage of the fixed schedule, or the end of
+  // each sub-step of the schedule for the adaptive solve (which interpolates)
+  const int nst = c->surr_adaptive ? c->surr_nsub : tgr::kSurrStages * c->surr_nsub;
   // sub-step sizes: geometric with ratio surr_grade (1: equal), summing to dt
   double hsub[tgr::kSurrMaxSub] = {0.0};
   {
@@ -543,8 +559,9 @@ int run_surrogate(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, 
     static const double cst[tgr::kSurrStages] = {0.43586652150845899942, 0.5 * (1.0 + 0.43586652150845899942), 1.0};
     double t0 = 0.0;
     for (int k = 0; k < nst; ++k) {
-      const int i = k / tgr::kSurrStages, j = k % tgr::kSurrStages;
-      if (j == 0 && i > 0) t0 += hsub[i - 1];
+      const int i = c->surr_adaptive ? k : k / tgr::kSurrStages;
+      const int j = c->surr_adaptive ? tgr::kSurrStages - 1 : k % tgr::kSurrStages;
+      if ((c->surr_adaptive || j == 0) && i > 0) t0 += hsub[i - 1];
       const double tau = t0 + cst[j] * hsub[i];
       const float v = (float)((std::log10(tau) - c->h_in_shift[col]) * c->h_in_scale[col]);
       if (tgr_launch_surr_set_column(X, rows, c->net_in_pad, col, v, s)) return fail(TGR_E_HIP, "column launch");
@@ -556,6 +573,21 @@ int run_surrogate(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, 
   P.s = p;
   P.s.queue = c->d_queue;
   P.s.ws = (double*)c->b_ws.p;
+  P.adaptive = c->surr_adaptive;
+  P.max_att = c->surr_max_att;
+  P.h0_fac = c->surr_h0_fac;
+  P.ntau = nst;
+  {
+    // times of the network outputs, in the order they were computed
+    static const double cs[tgr::kSurrStages] = {0.43586652150845899942, 0.5 * (1.0 + 0.43586652150845899942), 1.0};
+    double t0 = 0.0;
+    for (int k = 0; k < nst; ++k) {
+      const int i = c->surr_adaptive ? k : k / tgr::kSurrStages;
+      const int j = c->surr_adaptive ? tgr::kSurrStages - 1 : k % tgr::kSurrStages;
+      if ((c->surr_adaptive || j == 0) && i > 0) t0 += hsub[i - 1];
+      P.tau[k] = t0 + cs[j] * hsub[i];
+    }
+  }
   P.guess = guess;
   P.guess_stride = rows * c->net_out_pad;
   P.guess_ld = c->net_out_pad;

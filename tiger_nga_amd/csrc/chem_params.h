@@ -61,6 +61,11 @@ struct SurrParams {
   int nsub;                      // sub-steps (1..kSurrMaxSub)
   double hsub[kSurrMaxSub];      // their sizes (sum dt)
   int use_guess;                 // 0: Newton starts from the previous stage value
+  int adaptive;                  // 1: step-size control (sdirk_adaptive) instead of the fixed schedule
+  int max_att;                   // adaptive: attempted steps before the cell is rejected
+  double h0_fac;                 // adaptive: first step c / max_k |f0_k| wacc_k in [dt/1024, dt] (0: hsub[0])
+  int ntau;                      // number of network output times
+  double tau[kSurrStages * kSurrMaxSub];  // times of the network outputs, increasing
   int newton_max;
   double newton_tol;             // Newton stop: |correction| <= newton_tol x acceptance tolerance
   double tol_y, tol_t;           // acceptance: |error of Y_k| <= tol_y, |error of T| <= tol_t K

@@ -112,8 +112,27 @@ __device__ __forceinline__ int surr_cell(const SurrParams& P, const DevMech& m, 
   };
   const double wacc = lane < nsp ? 1.0 / (P.tol_y * rho) : (lane == nsp ? 1.0 / (P.tol_t * rho) : 0.0);
   Bdf<NEQ> b(m, tab, rl, zn_lds, ws, p.p_gp, p.rtol, p.atol, p.qmax, p.max_steps);
-  const int st =
-      b.sdirk_guided(sol, P.nsub, P.hsub, guess, P.use_guess != 0, wacc, P.newton_max, P.newton_tol, nit);
+  int st;
+  if (P.adaptive) {
+    // the network's state at time t: linear in t between its stage times
+    // (increment 0 at t = 0, the last output beyond the last time)
+    const int nt = P.ntau;
+    auto gat = [&](double t) -> double {
+      if (lane >= NEQ || !(t > 0.0)) return sol0;
+      int k = 0;
+      while (k < nt && P.tau[k] < t) ++k;
+      const double t1 = k < nt ? P.tau[k] : P.tau[nt - 1];
+      const double v1 = (double)gcell[(int64_t)(k < nt ? k : nt - 1) * P.guess_stride];
+      const double t0 = k > 0 ? P.tau[k - 1] : 0.0;
+      const double v0 = k > 0 ? (double)gcell[(int64_t)(k - 1) * P.guess_stride] : 0.0;
+      const double w = (k < nt && t1 > t0) ? (t - t0) / (t1 - t0) : 1.0;
+      return sol0 + rho * osc * (v0 + w * (v1 - v0));
+    };
+    st = b.sdirk_adaptive(sol, p.dt, P.hsub[0], P.h0_fac, gat, P.use_guess != 0, wacc, P.newton_max, P.newton_tol,
+                          P.max_att, nit);
+  } else {
+    st = b.sdirk_guided(sol, P.nsub, P.hsub, guess, P.use_guess != 0, wacc, P.newton_max, P.newton_tol, nit);
+  }
   nfe = b.c_nfe + b.c_nfe_dq;
   if (st) return st;
   // element-conserving projection: the least change of the species (weights
