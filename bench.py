@@ -146,16 +146,21 @@ def bench_surrogate(a, torch, fc, g, R_d, S0_d, S_d, HR_d, CS_d, dt, P, mech, sp
         return float(dY.max()), float(dY.pow(2).mean().sqrt()), float(dT.max())
 
     from tiger_nga_amd import flops
+    guess_t = int(fc.options.get("surr_guess", 0))  # the tuned (or default) network use
     fc.set_option("surr_probe", 0)
     t_forced = _time_steps(torch, a, guided, S_d, S0_d)
     S_d.copy_(S0_d)
     _, ss = fc.source_chemistry_surrogate(R_d, S_d, g, dt, P, P, HR_gp=HR_d, chemsrc_gp=CS_d, stream=sp)
     e_max, e_rms, eT = errors(Sp)
     state = S_d.clone()  # the guided result on the benchmark states, for the oracle comparison
-    fc.set_option("surr_guess", 0)
+    # the other network setting on the same states: what the network's
+    # guesses gain (or cost)
+    fc.set_option("surr_guess", 1 - guess_t)
+    t_other = _time_steps(torch, a, guided, S_d, S0_d)
     S_d.copy_(S0_d)
-    _, s0 = fc.source_chemistry_surrogate(R_d, S_d, g, dt, P, P, stream=sp)
-    fc.set_option("surr_guess", 1)
+    _, so = fc.source_chemistry_surrogate(R_d, S_d, g, dt, P, P, stream=sp)
+    o_max = errors(Sp)[0]
+    fc.set_option("surr_guess", guess_t)
     fc.set_option("surr_probe", 8)
     t_adapt = _time_steps(torch, a, guided, S_d, S0_d)
     # relaxed states: both paths
@@ -173,7 +178,14 @@ def bench_surrogate(a, torch, fc, g, R_d, S0_d, S_d, HR_d, CS_d, dt, P, mech, sp
     r_max, r_rms, rT = errors(Rp)
     fc.set_option("surr_probe", 8)
     fc.set_surrogate(None, None, None, None)
-    gf = flops.guided_flops(fc.info, ss, int(fc.options.get("surr_nsub", 4))) if ss["surrogate_ms"] > 0 else 0.0
+    gf = flops.guided_flops(fc.info, ss) if ss["surrogate_ms"] > 0 else 0.0
+
+    def net_case(t, st, emax):
+        return {"value": cells / t, "ms_per_step": t * 1e3, "accepted_fraction": st["accepted"] / max(1, cells),
+                "newton_iters_per_cell": st["newton_iters"] / max(1, cells), "max_species_err": emax}
+    cur, oth = net_case(t_forced, ss, e_max), net_case(t_other, so, o_max)
+    with_net, without_net = (cur, oth) if guess_t else (oth, cur)
+    opts = {k: fc.options[k] for k in sorted(fc.options) if k.startswith("surr_")}
     return {
         "_state": state,
         "guided_roofline": {"bound": "mfma", "compute_unit": "FP64 vector ALU", "achieved": gf / (
@@ -189,16 +201,21 @@ def bench_surrogate(a, torch, fc, g, R_d, S0_d, S_d, HR_d, CS_d, dt, P, mech, sp
                                        "enthalpy": ss["rej_enthalpy"]},
                           "guided_ms": ss["surrogate_ms"], "fallback_ms": ss["fallback_ms"],
                           "newton_iters_per_cell": ss["newton_iters"] / max(1, cells),
-                          "newton_iters_per_cell_without_network": s0["newton_iters"] / max(1, cells)},
+                          "substeps_per_cell": ss["substeps"] / max(1, cells)},
+        "options": opts,
+        "network": {"used": bool(guess_t), "with_network_guess": with_net, "without_network_guess": without_net,
+                    "gain": without_net["ms_per_step"] / with_net["ms_per_step"]},
         "max_species_err": e_max, "rms_species_err": e_rms, "max_T_err_K": eT,
         "error_reference": "plain FP64 step of this run on the same states (rtol 1e-6, atol 1e-12)",
         "relaxed_states": {"what": "the same block after one plain step of dt (radical pools relaxed)",
                            "plain_value": cells / t_rp, "guided_value": cells / t_rg, "speedup": t_rp / t_rg,
                            "accepted_fraction": sr["accepted"] / max(1, cells), "max_species_err": r_max,
                            "rms_species_err": r_rms, "max_T_err_K": rT},
-        "method": "per cell: network stage guesses -> sub-steps of the 3-stage L-stable SDIRK3(2), FP64 Newton, "
-                  "embedded error <= 1e-6 in Y, element-conserving projection, enthalpy-consistent T; rejected "
-                  "cells by the plain FP64 step"}
+        "method": "per cell: 3-stage L-stable SDIRK3(2) with one frozen FP64 Jacobian, %s sub-steps, FP64 "
+                  "Newton from %s, embedded error <= surr_tol_y in Y, element-conserving projection, "
+                  "enthalpy-consistent T; rejected cells by the plain FP64 step"
+                  % ("error-controlled" if fc.options.get("surr_adaptive", 1) else "fixed",
+                     "the network's guesses" if guess_t else "the derivative predictor")}
 
 
 def bench_properties(a, torch, fc, g, S_d, R_d, P, n, cells_label, mech, sp):

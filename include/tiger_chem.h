@@ -133,17 +133,29 @@ int tgr_chem_temperature_device(tgr_chem *ctx, int64_t n, const double *Y, const
                                 double *T, int *iters, void *stream);
 
 /* ---- surrogate-guided step (no counterpart in the reference) -------------
- * The learned part: a perceptron (include/tiger_mlp.h) trained on (state, dt)
- * -> increment pairs that the plain FP64 step generates predicts every cell's
- * state at the stage times of a one-step, L-stable SDIRK solve (two stages,
- * gamma = 1 - 1/sqrt 2, `surr_nsub` sub-steps, one difference-quotient
- * Jacobian and one LU per cell); FP64 Newton iterations start from those
- * predictions.  A cell is accepted when every Newton solve converges, the
- * embedded error estimate (filtered by (I - gamma h J)^-1) stays within
- * |dY_k| <= surr_tol_y and |dT| <= surr_tol_t, and the temperature implied by
- * conservation of the mixture enthalpy agrees with the solved one within
- * surr_tol_t; the species are projected back onto exact conservation of each
- * element's mass and the enthalpy-consistent temperature is written.  Rejected
+ * Every cell is integrated over dt by a one-step, L-stable, stiffly accurate
+ * SDIRK3(2) method (Alexander's three stages, gamma = 0.4358665215) with one
+ * difference-quotient Jacobian per cell, frozen over the call, and FP64
+ * simplified Newton per stage.  Two schedules: "surr_adaptive" 1 (default)
+ * controls the step size by the embedded error estimate (filtered by
+ * (I - gamma h J)^-1, accepted when |e_k| <= surr_tol_y in Y and surr_tol_t in
+ * T; next step x min(4, max(0.2, 0.9 e^-1/3)); a stage whose Newton iteration
+ * does not converge retries the step at h/4) from a first step of
+ * dt (r - 1) / (r^n - 1), r = "surr_grade", n = "surr_nsub", for at most
+ * "surr_max_att" attempted steps; "surr_adaptive" 0 takes exactly the n
+ * geometric sub-steps and accepts the cell when every sub-step's estimate
+ * passes.  The learned part: a perceptron (include/tiger_mlp.h) trained on
+ * (state, dt) -> increment pairs that the plain FP64 step generates predicts
+ * the state at the schedule's stage (fixed) or sub-step-end (adaptive)
+ * times; with "surr_guess" 1 each stage's Newton iteration starts from the
+ * previous stage value moved by the prediction's change, with 0 (default)
+ * from the stage's known part plus gamma h times the previous derivative
+ * (adaptive) or from the previous stage value (fixed) -- on GRI-3.0 64^3 the
+ * derivative start needs 13 % fewer iterations than the network's, so the
+ * network is used where the tuning record says it pays.  An accepted cell's
+ * species are projected back onto exact conservation of each element's mass
+ * and the temperature implied by conservation of the mixture enthalpy is
+ * written when it agrees with the solved one within surr_tol_t.  Rejected
  * cells are integrated by the plain FP64 step on the same call, so every cell
  * ends either accepted or exactly as tgr_chem_source_chemistry_device leaves
  * it.  Adaptive use: while the accepted fraction of the last guided call is
@@ -157,6 +169,7 @@ typedef struct {
   int64_t accepted, rejected;     /* cells accepted by the guided solve / handed to the plain step */
   int64_t newton_iters, rhs_evals; /* of the guided solve (Jacobian columns included) */
   int64_t rej_newton, rej_error, rej_enthalpy; /* why cells were rejected */
+  int64_t substeps;               /* attempted sub-steps of the guided solve (one LU each) */
   double surrogate_ms, fallback_ms; /* device time of features + network + guided solve, of the plain step */
   int guided;                     /* 1: the guided solve ran; 0: the adaptive policy sent the call to the plain step */
 } tgr_surrogate_stats;
@@ -195,14 +208,18 @@ int tgr_chem_source_chemistry_surrogate(tgr_chem *ctx, double dt, double p_therm
  * it; the context owns that network and frees it in tgr_chem_destroy. */
 int tgr_chem_load_surrogate(tgr_chem *ctx, const char *path);
 
-/* Real-valued knobs of the surrogate step: "surr_tol_y" (default 2e-6: the
- * embedded estimate measures the second-order companion solution, the
- * accepted third-order one is more accurate -- on GRI-3.0 64^3 its largest
- * species error against the plain step is below 6e-7 at this setting),
+/* Real-valued knobs of the surrogate step: "surr_tol_y" (default 2.5e-7: the
+ * adaptive schedule's error grows with its number of steps, each within the
+ * threshold; at this setting its largest species error against a 1e-10
+ * tolerance solve stays below 9e-7 on the H2, GRI-3.0, EthyRedF and DME39
+ * benchmark states, at 1e-6 it reaches 3.6e-6 on H2 -- the tuning record
+ * holds the largest safe value per mechanism, 1e-6 for GRI-3.0),
  * "surr_tol_t" (K, default 1e-2), "surr_newton_tol" (Newton stops when the
  * correction is below this fraction of the acceptance tolerance, default 0.1),
- * "surr_grade" (ratio of consecutive sub-step sizes, default 1),
- * "surr_min_accept" (adaptive-use threshold, default 0.15). */
+ * "surr_grade" (ratio of consecutive sub-step sizes, default 2),
+ * "surr_min_accept" (adaptive-use threshold, default 0.15), "surr_h0_fac"
+ * (adaptive schedule: first step h0_fac / max_k |f_k(y0)| / tol_k within
+ * [dt/1024, dt] instead of the first sub-step; default 0 = off). */
 int tgr_chem_set_option_real(tgr_chem *ctx, const char *key, double value);
 
 /* Tuning knobs (no counterpart in the reference):
@@ -213,10 +230,13 @@ int tgr_chem_set_option_real(tgr_chem *ctx, const char *key, double value);
  *                 straight-line code of the mechanism), 0 one state per wavefront
  *   "max_order"   BDF maximum order 1..5 (CVODE default 5)
  *   "max_steps"   MAX_NSTEPS
- *   "surr_nsub"   sub-steps of the surrogate-guided solve, 1..8 (default 4)
- *   "surr_guess"  1 (default): Newton from the network's prediction; 0: from
- *                 the sub-step's initial state (the same solve without the network)
- *   "surr_newton_max"  Newton iterations per stage before the cell is rejected (default 8)
+ *   "surr_nsub"   sub-steps of the surrogate-guided schedule, 1..8 (default 8)
+ *   "surr_guess"  1: Newton from the network's prediction; 0 (default): without
+ *                 the network (derivative or previous-stage start, see above)
+ *   "surr_adaptive"  1 (default): error-controlled step sizes; 0: the fixed schedule
+ *   "surr_max_att"   adaptive schedule: attempted steps before the cell is
+ *                    rejected, 1..1000 (default 48)
+ *   "surr_newton_max"  Newton iterations per stage before the stage fails (default 8)
  *   "surr_probe"  adaptive use: a guided probe every this many calls while the
  *                 guided solve is not paying (default 8; 0 = always guided) */
 int tgr_chem_set_option(tgr_chem *ctx, const char *key, int64_t value);

@@ -8,13 +8,13 @@ Choices (all exposed as context options of include/tiger_chem.h):
 * ``grid``         workgroups of the persistent step kernel: the resident
   capacity (occupancy x CUs) or a fraction of it (fewer workgroups leave CUs
   idle but reduce contention for the L2 and the work counter at small sizes);
-* ``surr_nsub`` / ``surr_grade``  sub-step schedule of the surrogate-guided
-  solve (when a trained network exists), chosen on relaxed states (after one
-  plain step, where the guided solve pays) for throughput among the schedules
-  whose max species error against the plain step stays <= 1e-6;
-* ``surr_tol_y``  its acceptance threshold: the largest of 1e-6, 2e-6, 4e-6
-  whose measured max species error stays <= 5e-7 (half the 1e-6 bound) on
-  relaxed and on freshly generated states.
+* the surrogate-guided solve (when a trained network exists): schedule
+  (``surr_adaptive``, ``surr_nsub``, ``surr_grade``, ``surr_max_att``),
+  network use (``surr_guess``) and acceptance threshold (``surr_tol_y``
+  2.5e-7 .. 2e-6): the fastest combination, timed on freshly generated plus relaxed
+  states (the block after one plain step), whose max species error against
+  the plain step stays <= 5e-7 (half the 1e-6 bound: larger blocks reach
+  further into the tail) on both.
 
 Results are cached in ``tiger_nga_amd/tuning.json`` keyed by mechanism, device
 name and the sample size bucket; ``apply(fc)`` sets the cached choices on a
@@ -33,6 +33,12 @@ from . import inputs
 HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuning.json")
 WORKLOAD = {"h2": "h2", "gri30": "gri30", "dme39": "dme39", "ethyred": "ethyred"}
+# (adaptive, sub-steps, grade, attempt budget) of the guided solve
+SURR_SCHEDULES = ((1, 8, 2.0, 48), (1, 4, 1.0, 32), (1, 8, 3.0, 64), (0, 4, 1.0, 48), (0, 8, 1.5, 48))
+# acceptance thresholds: the adaptive schedule's error against the plain step
+# grows with the number of steps it takes (each within the threshold), so the
+# mechanisms that take many need the smaller ones
+SURR_TOLS = (2.5e-7, 5e-7, 1e-6, 2e-6)
 
 
 def _device_name():
@@ -97,64 +103,62 @@ def tune(mech: str, cells: int = 32 ** 3, seed: int = 7, reps: int = 2, log=prin
     rec["grid"] = best_grid
     rec["measured"]["step_ms"] = {str(k): v * 1e3 for k, v in gt.items()}
     fc.set_option("grid", best_grid)
-    # surrogate schedule, measured on relaxed states (the block after one
-    # plain step): where the guided solve is used at all -- on freshly
-    # generated states the adaptive policy keeps the plain step
+    # surrogate-guided solve: schedule, network use and acceptance threshold,
+    # measured on freshly generated states and on relaxed ones (the block
+    # after one plain step, as in a running simulation)
     path = surrogate.model_path(mech)
     if os.path.exists(path):
-        S.copy_(S0)
-        fc.source_chemistry(R_d, S, g, dt, Pg, Pg)
-        S0 = S.clone()
-        fc.source_chemistry(R_d, S, g, dt, Pg, Pg)
-        ref = S.clone()
         nsp = fc.nsp
+        sets = []
+        for relax in (0, 1):
+            Rb, Sb0, gb, Pb = surrogate.block_states(fc, WORKLOAD[mech], cells, seed, relax)
+            Sb = Sb0.clone()
+
+            def pstep(Rb=Rb, Sb=Sb, Sb0=Sb0, gb=gb, Pb=Pb):
+                Sb.copy_(Sb0)
+                fc.source_chemistry(Rb, Sb, gb, dt, Pb, Pb, sync=False)
+            tp = _time(torch, pstep, reps)
+            pstep()
+            sets.append((Rb, Sb0, gb, Pb, Sb.clone(), tp))
+        rec["measured"]["plain_ms"] = {"generated": sets[0][5] * 1e3, "relaxed": sets[1][5] * 1e3}
         sur = surrogate.Surrogate.load(path)
         sur.attach(fc)
         fc.set_option("surr_probe", 0)
-        best = None
         st_all = {}
-        for nsub, grade in ((2, 1.0), (4, 1.0), (6, 1.0), (8, 1.5)):
-            fc.set_option("surr_nsub", nsub)
-            fc.set_option("surr_grade", grade)
+        best = None
+        for (adaptive, nsub, grade, att) in SURR_SCHEDULES:
+            for guess in (0, 1):
+                for tol in SURR_TOLS:
+                    fc.set_option("surr_adaptive", adaptive)
+                    fc.set_option("surr_nsub", nsub)
+                    fc.set_option("surr_grade", grade)
+                    fc.set_option("surr_max_att", att)
+                    fc.set_option("surr_guess", guess)
+                    fc.set_option("surr_tol_y", tol)
+                    t_tot, errs = 0.0, []
+                    for (Rb, Sb0, gb, Pb, ref, _) in sets:
+                        Sb = Sb0.clone()
 
-            def sstep():
-                S.copy_(S0)
-                fc.source_chemistry_surrogate(R_d, S, g, dt, Pg, Pg, sync=False)
-            t = _time(torch, sstep, reps)
-            err = float((S[..., :nsp] - ref[..., :nsp]).abs().max())
-            st_all["%d/%.1f" % (nsub, grade)] = {"ms": t * 1e3, "max_species_err": err}
-            log("  surrogate nsub %d grade %.1f: %.2f ms, max |dY| %.2e" % (nsub, grade, t * 1e3, err))
-            if err <= 1e-6 and (best is None or t < best[0]):
-                best = (t, nsub, grade)
+                        def sstep(Rb=Rb, Sb=Sb, Sb0=Sb0, gb=gb, Pb=Pb):
+                            Sb.copy_(Sb0)
+                            fc.source_chemistry_surrogate(Rb, Sb, gb, dt, Pb, Pb, sync=False)
+                        t_tot += _time(torch, sstep, reps)
+                        sstep()
+                        torch.cuda.synchronize()
+                        errs.append(float((Sb[..., :nsp] - ref[..., :nsp]).abs().max()))
+                    name = "%s n%d r%.1f a%d guess%d tol%.1e" % ("adaptive" if adaptive else "fixed", nsub, grade, att,
+                                                               guess, tol)
+                    st_all[name] = {"ms_generated_plus_relaxed": t_tot * 1e3, "max_species_err_generated": errs[0],
+                                    "max_species_err_relaxed": errs[1]}
+                    log("  surrogate %s: %.2f ms, max |dY| %.2e / %.2e" % (name, t_tot * 1e3, errs[0], errs[1]))
+                    # half the 1e-6 bound: larger blocks reach further into the tail
+                    if max(errs) <= 5e-7 and (best is None or t_tot < best[0]):
+                        best = (t_tot, adaptive, nsub, grade, att, guess, tol)
         rec["measured"]["surrogate"] = st_all
         if best:
-            rec["surr_nsub"], rec["surr_grade"] = best[1], best[2]
-            fc.set_option("surr_nsub", best[1])
-            fc.set_option("surr_grade", best[2])
-            # acceptance threshold: the largest whose measured max species
-            # error stays <= 5e-7 on both the relaxed and the generated states
-            # (half the 1e-6 bound: larger blocks reach further into the tail)
-            R0, F0, g0, P0 = surrogate.block_states(fc, WORKLOAD[mech], cells, seed, 0)
-            fc.set_surrogate(None, None, None, None)
-            ref0 = F0.clone()
-            fc.source_chemistry(R0, ref0, g0, dt, P0, P0)
-            sur.attach(fc)
-            fc.set_option("surr_probe", 0)
-            tol_ok = 1e-6
-            for tol in (1e-6, 2e-6, 4e-6):
-                fc.set_option("surr_tol_y", tol)
-                errs = []
-                for (Rb, Sb0, gb, Pb, rb) in ((R_d, S0, g, Pg, ref), (R0, F0, g0, P0, ref0)):
-                    Sb = Sb0.clone()
-                    fc.source_chemistry_surrogate(Rb, Sb, gb, dt, Pb, Pb)
-                    errs.append(float((Sb[..., :nsp] - rb[..., :nsp]).abs().max()))
-                st_all["tol_y %.0e" % tol] = {"max_species_err_relaxed": errs[0], "max_species_err_generated": errs[1]}
-                log("  surrogate tol_y %.0e: max |dY| relaxed %.2e, generated %.2e" % (tol, errs[0], errs[1]))
-                if max(errs) <= 5e-7:
-                    tol_ok = tol
-                else:
-                    break
-            rec["surr_tol_y"] = tol_ok
+            (_, rec["surr_adaptive"], rec["surr_nsub"], rec["surr_grade"], rec["surr_max_att"], rec["surr_guess"],
+             rec["surr_tol_y"]) = best
+            rec["measured"]["surrogate_speedup"] = (sets[0][5] + sets[1][5]) / best[0]
         fc.set_surrogate(None, None, None, None)
     fc.close()
     return rec
@@ -190,7 +194,8 @@ def apply(fc, cells: int, path=CACHE) -> dict:
     done = {}
     if not rec:
         return done
-    for k in ("rhs_mapping", "grid", "surr_nsub", "surr_grade", "surr_tol_y"):
+    for k in ("rhs_mapping", "grid", "surr_adaptive", "surr_nsub", "surr_grade", "surr_max_att", "surr_guess",
+              "surr_tol_y"):
         if k in rec:
             fc.set_option(k, rec[k])
             done[k] = rec[k]
@@ -199,7 +204,7 @@ def apply(fc, cells: int, path=CACHE) -> dict:
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--mech", nargs="*", default=["h2", "gri30", "dme39"])
+    ap.add_argument("--mech", nargs="*", default=["h2", "gri30", "dme39", "ethyred"])
     ap.add_argument("--cells", type=int, default=32 ** 3)
     ap.add_argument("--cache", default=CACHE)
     a = ap.parse_args(argv)

@@ -100,15 +100,15 @@ struct tgr_chem {
   DevBuf b_x, b_guess, b_rej;
   unsigned int* d_rej_count = nullptr;
   unsigned long long* d_stotals = nullptr;
-  int surr_nsub = 4, surr_guess = 1, surr_newton_max = 8;
+  int surr_nsub = 8, surr_guess = 0, surr_newton_max = 8;
   // adaptive use: while the accepted fraction of the last guided call is
   // below surr_min_accept, calls go to the plain step, with a guided probe
   // every surr_probe calls (0: always guided)
   int surr_probe = 8, surr_skipped = 0;
-  int surr_adaptive = 0, surr_max_att = 48;
+  int surr_adaptive = 1, surr_max_att = 48;
   double surr_h0_fac = 0.0;
   double surr_min_accept = 0.15, surr_last_accept = 1.0;
-  double surr_newton_tol = 0.1, surr_tol_y = 2.0e-6, surr_tol_t = 1.0e-2, surr_grade = 1.0;
+  double surr_newton_tol = 0.1, surr_tol_y = 2.5e-7, surr_tol_t = 1.0e-2, surr_grade = 2.0;
   hipEvent_t ev2 = nullptr;
 };
 
@@ -648,6 +648,7 @@ int run_surrogate(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, 
       sstats->rej_newton = st[4];
       sstats->rej_error = st[5];
       sstats->rej_enthalpy = st[6];
+      sstats->substeps = st[7];
       sstats->surrogate_ms = ms_s;
       sstats->fallback_ms = ms_p;
     }

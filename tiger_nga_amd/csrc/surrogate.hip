@@ -89,7 +89,7 @@ __global__ void surr_reset_kernel(unsigned int* queue, unsigned int* rej_count, 
 template <int NEQ>
 __device__ __forceinline__ int surr_cell(const SurrParams& P, const DevMech& m, tchar* tab, TGR_LDS RhsLds<NEQ>* rl,
                                          int64_t cell, TGR_LDS double (&zn_lds)[kQmaxAbs + 2][kWave], double* ws,
-                                         int& nit, int& nfe) {
+                                         int& nit, int& nfe, int& nlu) {
   const StepParams& p = P.s;
   const int lane = lane_id();
   const int nsp = NEQ - 1;
@@ -134,6 +134,7 @@ __device__ __forceinline__ int surr_cell(const SurrParams& P, const DevMech& m, 
     st = b.sdirk_guided(sol, P.nsub, P.hsub, guess, P.use_guess != 0, wacc, P.newton_max, P.newton_tol, nit);
   }
   nfe = b.c_nfe + b.c_nfe_dq;
+  nlu = b.c_nlu;
   if (st) return st;
   // element-conserving projection: the least change of the species (weights
   // |rho Y_k|) that restores every element's mass to its initial value
@@ -247,9 +248,9 @@ __global__ __launch_bounds__(kSurrBlock) void surr_kernel(SurrParams P) {
     if (lane == 0) c = atomicAdd(P.s.queue, 1u);
     const int64_t cell = (int64_t)(unsigned int)__builtin_amdgcn_readfirstlane((int)c);
     if (cell >= P.s.ncell) break;
-    int nit = 0, nfe = 0;
+    int nit = 0, nfe = 0, nlu = 0;
     const int st = surr_cell<NEQ>(P, m, (tchar*)tab_s, (TGR_LDS RhsLds<NEQ>*)&rl[w], cell,
-                                  *(TGR_LDS double(*)[kQmaxAbs + 2][kWave]) & zn_s[w], ws, nit, nfe);
+                                  *(TGR_LDS double(*)[kQmaxAbs + 2][kWave]) & zn_s[w], ws, nit, nfe, nlu);
     if (st < 0) continue;
     if (lane == 0) {
       if (st != 0) {
@@ -260,6 +261,7 @@ __global__ __launch_bounds__(kSurrBlock) void surr_kernel(SurrParams P) {
       atomicAdd(&P.stotals[2], (unsigned long long)nit);
       atomicAdd(&P.stotals[3], (unsigned long long)nfe);
       if (st > 0) atomicAdd(&P.stotals[3 + st], 1ull);
+      atomicAdd(&P.stotals[7], (unsigned long long)nlu);
     }
   }
 }

@@ -41,7 +41,8 @@ class _Stats(ctypes.Structure):
 
 class _SurrStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
-        "accepted", "rejected", "newton_iters", "rhs_evals", "rej_newton", "rej_error", "rej_enthalpy")] + [
+        "accepted", "rejected", "newton_iters", "rhs_evals", "rej_newton", "rej_error", "rej_enthalpy",
+        "substeps")] + [
         ("surrogate_ms", ctypes.c_double), ("fallback_ms", ctypes.c_double), ("guided", ctypes.c_int)]
 
 

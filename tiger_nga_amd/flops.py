@@ -68,16 +68,18 @@ def properties_flops(nsp: int) -> float:
     return 3 * nsp + 19 * nsp + 10 * nsp * nsp + 12 * nsp + 5 + nsp
 
 
-def guided_flops(info: dict, sstats: dict, nsub: int) -> float:
+def guided_flops(info: dict, sstats: dict, nsub: int = 0) -> float:
     """Algorithmic flops of the surrogate step's guided solve (before the plain
     fallback) from its counters: every right-hand side (Newton and
     difference-quotient columns), one LU of I - gamma h J and one filtered
-    error estimate per sub-step of every attempted cell, a triangular solve
-    and update per Newton iteration, and the enthalpy check / element
-    projection (taken as 60 neq per cell)."""
+    error estimate per attempted sub-step (the counted ones; without that
+    counter, nsub per attempted cell), a triangular solve and update per
+    Newton iteration, and the enthalpy check / element projection (taken as
+    60 neq per cell)."""
     n = info["nsp"] + 1
     cells = sstats["accepted"] + sstats["rejected"]
     f_lu = 2.0 * n * n + 2.0 / 3.0 * n ** 3
     f_solve = 2.0 * n * n + 4.0 * n
-    return (sstats["rhs_evals"] * rhs_flops(info) + cells * nsub * (f_lu + f_solve)
+    subs = sstats.get("substeps") or cells * nsub
+    return (sstats["rhs_evals"] * rhs_flops(info) + subs * (f_lu + f_solve)
             + sstats["newton_iters"] * f_solve + cells * 60.0 * n)
