@@ -42,8 +42,11 @@ SURR_TOLS = (2.5e-7, 5e-7, 1e-6, 2e-6)
 
 
 def _device_name():
+    """Architecture and CU count (the marketing name depends on which ids
+    table the runtime finds, and differs under a profiler)."""
     import torch
-    return torch.cuda.get_device_name(0)
+    p = torch.cuda.get_device_properties(0)
+    return "%s/%dCU" % (getattr(p, "gcnArchName", "gpu").split(":")[0], p.multi_processor_count)
 
 
 def _key(mech, cells):
