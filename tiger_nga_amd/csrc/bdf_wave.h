@@ -65,7 +65,7 @@ struct Bdf {
   bool act;
   // per-lane vectors; the Nordsieck history lives in LDS (lane-private
   // columns), so that zn[q] with a run-time order q is a plain address
-  TGR_LDS double (&zn)[kQmaxAbs + 2][kWave];
+  TGR_LDS double (*zn)[kWave];  // [kQmaxAbs + 2] rows
   double ewt, acor, y, tempv, ftemp;
   double* jw;  // saved Jacobian: jw[c * 64 + lane] = J(lane, c)
   double M[NEQ];
@@ -91,6 +91,17 @@ struct Bdf {
     n = NEQ;
     act = lane < NEQ;
     qmax = (qm >= 1 && qm <= kQmaxAbs) ? qm : kQmaxAbs;
+  }
+
+  // without the Nordsieck storage: for the one-step SDIRK solves only (the
+  // BDF members that use zn, tq, l or tau must not be called)
+  __device__ Bdf(const DevMech& mech, tchar* tabl, TGR_LDS RhsLds<NEQ>* rl, double* ws, double pgp, double rt, double at)
+      : m(mech), tab(tabl), S(rl), p_gp(pgp), rtol(rt), atol(at), max_steps(0), zn(nullptr), jw(ws) {
+    tq = l = tau = nullptr;
+    lane = lane_id();
+    n = NEQ;
+    act = lane < NEQ;
+    qmax = kQmaxAbs;
   }
 
   __device__ __forceinline__ double f(double yv) { return rhs_call<NEQ>(tab, S, m.tmid, act ? yv : 0.0, p_gp); }

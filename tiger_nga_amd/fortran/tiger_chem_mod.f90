@@ -31,7 +31,8 @@ module tiger_chem
   end type tgr_stats
 
   public :: tgr_chem_create, tgr_chem_destroy, tgr_chem_source_chemistry, tgr_chem_source_chemistry_device
-  public :: tgr_chem_molar_mass, tgr_chem_species_name, tgr_chem_set_option, tgr_chem_last_error
+  public :: tgr_chem_molar_mass, tgr_chem_species_name, tgr_chem_set_option, tgr_chem_set_option_real
+  public :: tgr_chem_last_error
   public :: tgr_chem_properties_device, tgr_chem_temperature_device
   public :: tgr_chem_load_surrogate, tgr_chem_source_chemistry_surrogate
   public :: tiger_chem_load_surrogate, tiger_chem_source_chemistry_surrogate
@@ -71,6 +72,13 @@ module tiger_chem
        character(kind=c_char), dimension(*), intent(in) :: key
        integer(c_int64_t), value :: val
      end function tgr_chem_set_option
+
+     integer(c_int) function tgr_chem_set_option_real(ctx, key, val) bind(C, name="tgr_chem_set_option_real")
+       import :: c_int, c_ptr, c_char, c_double
+       type(c_ptr), value :: ctx
+       character(kind=c_char), dimension(*), intent(in) :: key
+       real(c_double), value :: val
+     end function tgr_chem_set_option_real
 
      ! host arrays; mask, HR_gp, chemsrc_gp, stats may be c_null_ptr
      integer(c_int) function tgr_chem_source_chemistry(ctx, dt, p_thermo, p_thermo_old, g, rho, sc, mask, hr, &
