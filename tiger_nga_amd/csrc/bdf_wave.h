@@ -724,9 +724,11 @@ struct Bdf {
   // The same method with step-size control: from h = h0 (or, with h0_fac > 0,
   // h0_fac / max_k |f_k(y0)| wacc_k within [dt/1024, dt]), each step's error
   // estimate e (as above) accepts the step when e <= 1 and scales the next
-  // step by min(4, max(0.2, 0.9 e^(-1/3))), a failed step is retried with the
-  // smaller h; the last step is cut to end at dt.  One Jacobian at the initial
-  // state, one LU per attempted step.  gat(t) is the network's prediction of
+  // step by min(grow, max(0.2, 0.9 e^(-1/3))) -- after an accepted step a
+  // factor in [1, keep) keeps h, so that the LU factors of I - gamma h J are
+  // reused -- a failed step is retried with the smaller h; the last step is
+  // cut to end at dt.  One Jacobian at the initial state, one LU per change
+  // of h.  gat(t) is the network's prediction of
   // the state at time t; each stage's Newton starts from the previous stage
   // value moved by the prediction's change between the two stage times;
   // without a network each stage starts from its known part plus gamma h
@@ -738,7 +740,7 @@ struct Bdf {
   template <class GuessAt>
   __device__ __forceinline__ int sdirk_adaptive(double& yv, double dt, double h0, double h0_fac, const GuessAt& gat,
                                                 bool use_guess, double wacc, int nmax, double ntol, int max_att,
-                                                int& nit) {
+                                                double grow, double keep, int& nit) {
     c_nfe = c_nfe_dq = c_nje = c_nsetups = c_nni = c_netf = c_ncfn = c_nlu = 0;
     nst = 0;
     const double y0 = act ? yv : 0.0;
@@ -758,6 +760,7 @@ struct Bdf {
     double yi = y0, t = 0.0, hh = fmin(h0, dt);
     double kl = act ? f0 : 0.0;  // derivative at yi (the last stage's, the method being stiffly accurate)
     int why = 2;  // reason of the last failed attempt: 1 Newton / singular, 2 error test
+    double h_lu = -1.0;  // step size of the factors held in M (-1: none)
     for (int att = 0; att < max_att; ++att) {
       if (dt - t <= 1.0e-12 * dt) {
         yv = yi;
@@ -765,7 +768,7 @@ struct Bdf {
       }
       if (t + hh > dt || dt - (t + hh) < 1.0e-3 * hh) hh = dt - t;
       const double gh = kSdG * hh;
-      {
+      if (hh != h_lu) {
         const TGR_GLOBAL double* jr = fresh(jw) + lane;
 #pragma unroll
         for (int c = 0; c < NEQ; ++c) {
@@ -773,12 +776,14 @@ struct Bdf {
           if (c == lane) v += 1.0;
           M[c] = v;
         }
-      }
-      c_nlu++;
-      if (getrf()) {
-        hh *= 0.25;  // singular iteration matrix: retry with a smaller step
-        why = 1;
-        continue;
+        c_nlu++;
+        h_lu = -1.0;
+        if (getrf()) {
+          hh *= 0.25;  // singular iteration matrix: retry with a smaller step
+          why = 1;
+          continue;
+        }
+        h_lu = hh;
       }
       const double rg = 1.0 / gh;
       const double ts[3] = {t + kSdG * hh, t + kSdC2 * hh, t + hh};
@@ -829,12 +834,13 @@ struct Bdf {
           getrs(act ? hh * ((kSdB1 - kSdBh1) * k1 + (kSdB2 - kSdBh2) * k2 + kSdG * k3) : 0.0);
       const double en = wave_max(act ? fabs(e) * wacc : 0.0);
       if (!isfinite(en)) return 1;
-      const double fac = fmin(4.0, fmax(0.2, 0.9 * rpow(fmax(en, 1.0e-10), -1.0 / 3.0)));
+      double fac = fmin(grow, fmax(0.2, 0.9 * rpow(fmax(en, 1.0e-10), -1.0 / 3.0)));
       if (en <= 1.0) {
         nst++;
         t += hh;
         yi = z;
         kl = k3;
+        if (fac >= 1.0 && fac < keep) fac = 1.0;  // keep h and reuse the factors
       } else {
         why = 2;
       }

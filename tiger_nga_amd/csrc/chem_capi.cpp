@@ -106,7 +106,7 @@ struct tgr_chem {
   // every surr_probe calls (0: always guided)
   int surr_probe = 8, surr_skipped = 0;
   int surr_adaptive = 1, surr_max_att = 48;
-  double surr_h0_fac = 0.0;
+  double surr_h0_fac = 0.0, surr_grow = 10.0, surr_keep = 1.5;
   double surr_min_accept = 0.15, surr_last_accept = 1.0;
   double surr_newton_tol = 0.1, surr_tol_y = 2.5e-7, surr_tol_t = 1.0e-2, surr_grade = 2.0;
   hipEvent_t ev2 = nullptr;
@@ -460,7 +460,13 @@ int tgr_chem_set_option_real(tgr_chem* c, const char* key, double v) {
     return 0;
   }
   if (!(v > 0.0)) return fail(TGR_E_ARG, "value must be positive and finite");
-  if (!strcmp(key, "surr_tol_y")) c->surr_tol_y = v;
+  if (!strcmp(key, "surr_grow")) {
+    if (v < 1.5 || v > 100.0) return fail(TGR_E_ARG, "surr_grow must be in [1.5, 100]");
+    c->surr_grow = v;
+  } else if (!strcmp(key, "surr_keep")) {
+    if (v < 1.0 || v > 4.0) return fail(TGR_E_ARG, "surr_keep must be in [1, 4]");
+    c->surr_keep = v;
+  } else if (!strcmp(key, "surr_tol_y")) c->surr_tol_y = v;
   else if (!strcmp(key, "surr_tol_t")) c->surr_tol_t = v;
   else if (!strcmp(key, "surr_newton_tol")) c->surr_newton_tol = v;
   else if (!strcmp(key, "surr_min_accept")) {
@@ -577,6 +583,8 @@ int run_surrogate(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, 
   P.max_att = c->surr_max_att;
   P.h0_fac = c->surr_h0_fac;
   P.ntau = nst;
+  P.grow = c->surr_grow;
+  P.keep = c->surr_keep;
   {
     // times of the network outputs, in the order they were computed
     static const double cs[tgr::kSurrStages] = {0.43586652150845899942, 0.5 * (1.0 + 0.43586652150845899942), 1.0};

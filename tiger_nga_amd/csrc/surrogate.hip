@@ -129,7 +129,7 @@ __device__ __forceinline__ int surr_cell(const SurrParams& P, const DevMech& m, 
       return sol0 + rho * osc * (v0 + w * (v1 - v0));
     };
     st = b.sdirk_adaptive(sol, p.dt, P.hsub[0], P.h0_fac, gat, P.use_guess != 0, wacc, P.newton_max, P.newton_tol,
-                          P.max_att, nit);
+                          P.max_att, P.grow, P.keep, nit);
   } else {
     st = b.sdirk_guided(sol, P.nsub, P.hsub, guess, P.use_guess != 0, wacc, P.newton_max, P.newton_tol, nit);
   }

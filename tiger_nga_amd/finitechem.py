@@ -39,6 +39,10 @@ class _Stats(ctypes.Structure):
         ("kernel_ms", ctypes.c_double)]
 
 
+REAL_OPTIONS = ("surr_tol_y", "surr_tol_t", "surr_newton_tol", "surr_grade", "surr_min_accept", "surr_h0_fac",
+                "surr_grow", "surr_keep")
+
+
 class _SurrStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
         "accepted", "rejected", "newton_iters", "rhs_evals", "rej_newton", "rej_error", "rej_enthalpy",
@@ -219,8 +223,9 @@ class FiniteChem:
     def set_option(self, key: str, value):
         """Integer knobs ("grid", "rhs_mapping", "max_order", "max_steps",
         "surr_nsub", "surr_guess", "surr_newton_max", "surr_probe", "surr_adaptive", "surr_max_att") or real ones
-        ("surr_tol_y", "surr_tol_t", "surr_newton_tol", "surr_grade", "surr_min_accept", "surr_h0_fac")."""
-        if key in ("surr_tol_y", "surr_tol_t", "surr_newton_tol", "surr_grade", "surr_min_accept", "surr_h0_fac"):
+        ("surr_tol_y", "surr_tol_t", "surr_newton_tol", "surr_grade", "surr_min_accept", "surr_h0_fac",
+        "surr_grow", "surr_keep")."""
+        if key in REAL_OPTIONS:
             _check(self.lib, self.lib.tgr_chem_set_option_real(self._h, key.encode(), float(value)), "set_option")
         else:
             _check(self.lib, self.lib.tgr_chem_set_option(self._h, key.encode(), int(value)), "set_option")
