@@ -178,7 +178,8 @@ def bench_surrogate(a, torch, fc, g, R_d, S0_d, S_d, HR_d, CS_d, dt, P, mech, sp
     r_max, r_rms, rT = errors(Rp)
     fc.set_option("surr_probe", 8)
     fc.set_surrogate(None, None, None, None)
-    gf = flops.guided_flops(fc.info, ss) if ss["surrogate_ms"] > 0 else 0.0
+    gf = (flops.guided_flops(fc.info, ss, split=bool(fc.options.get("surr_split_jac", 1)))
+          if ss["surrogate_ms"] > 0 else 0.0)
 
     def net_case(t, st, emax):
         return {"value": cells / t, "ms_per_step": t * 1e3, "accepted_fraction": st["accepted"] / max(1, cells),

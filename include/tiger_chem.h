@@ -135,8 +135,8 @@ int tgr_chem_temperature_device(tgr_chem *ctx, int64_t n, const double *Y, const
 /* ---- surrogate-guided step (no counterpart in the reference) -------------
  * Every cell is integrated over dt by a one-step, L-stable, stiffly accurate
  * SDIRK3(2) method (Alexander's three stages, gamma = 0.4358665215) with one
- * difference-quotient Jacobian per cell, frozen over the call, and FP64
- * simplified Newton per stage.  Two schedules: "surr_adaptive" 1 (default)
+ * difference-quotient Jacobian per cell ("surr_split_jac"), frozen over the
+ * call, and FP64 simplified Newton per stage.  Two schedules: "surr_adaptive" 1 (default)
  * controls the step size by the embedded error estimate (filtered by
  * (I - gamma h J)^-1, accepted when |e_k| <= surr_tol_y in Y and surr_tol_t in
  * T; next step x min(surr_grow, max(0.2, 0.9 e^-1/3)), where a factor in
@@ -242,6 +242,11 @@ int tgr_chem_set_option_real(tgr_chem *ctx, const char *key, double value);
  *   "surr_adaptive"  1 (default): error-controlled step sizes; 0: the fixed schedule
  *   "surr_max_att"   adaptive schedule: attempted steps before the cell is
  *                    rejected, 1..1000 (default 48)
+ *   "surr_split_jac" 1 (default): the guided solve's Jacobian takes its species
+ *                    columns as difference quotients at fixed temperature that
+ *                    reuse the rate constants and thermo of the base evaluation,
+ *                    plus df/dT times dT/dy (T = rho T / rho); 0: every column a
+ *                    full difference quotient as in the plain step
  *   "surr_newton_max"  Newton iterations per stage before the stage fails (default 8)
  *   "surr_probe"  adaptive use: a guided probe every this many calls while the
  *                 guided solve is not paying (default 8; 0 = always guided) */

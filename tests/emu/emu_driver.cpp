@@ -22,6 +22,13 @@
 #include "mech_h2.h"
 #include "tables_host.h"
 
+struct tgr_emu_jac_args {
+  long n;
+  double p_gp, rtol, atol, h;
+  const double* sol;
+  double *jdq, *jsplit, *f;
+};
+
 namespace {
 
 const tgr_mech_tables* find_mech(const char* name) {
@@ -94,7 +101,101 @@ void run(const tgr::StepParams& p, int* counters) {
   delete r;
 }
 
+// The two Jacobians of the one-step solves on packed states: Bdf::dq_jac
+// (every column a full difference quotient) and Bdf::jac_split (rate
+// constants evaluated once), J[c][row][col].
+template <int NEQ>
+struct JacRun {
+  const tgr_emu_jac_args* a;
+  tgr::DevMech m;
+  tgr::RhsLds<NEQ> rl;
+  tgr::RateCache<NEQ> rc;
+  std::vector<double> ws;
+  tgr_emu::Wave wave;
+  int finished = 0;
+
+  static void lane_main(unsigned hi, unsigned lo, int l) {
+    JacRun* r = (JacRun*)(((uintptr_t)hi << 32) | (uintptr_t)lo);
+    const tgr_emu_jac_args& a = *r->a;
+    for (long c = 0; c < a.n; ++c) {
+      const double yv = l < NEQ ? a.sol[c * NEQ + l] : 0.0;
+      tgr::Bdf<NEQ> b(r->m, r->m.tab, &r->rl, r->ws.data(), a.p_gp, a.rtol, a.atol);
+      b.set_ewt(yv);
+      b.h = a.h;
+      const double f0 = b.f(yv);
+      b.dq_jac(yv, f0);
+      if (l < NEQ)
+        for (int k = 0; k < NEQ; ++k) a.jdq[(c * NEQ + l) * NEQ + k] = r->ws[k * tgr::kWave + l];
+      b.rc = &r->rc;
+      const double fs = b.jac_split(yv);
+      if (l < NEQ) {
+        for (int k = 0; k < NEQ; ++k) a.jsplit[(c * NEQ + l) * NEQ + k] = r->ws[k * tgr::kWave + l];
+        a.f[c * NEQ + l] = fs - f0;  // the base evaluations agree
+      }
+    }
+    r->finished++;
+    if (r->finished == 64) {
+      setcontext(&r->wave.home);
+    } else {
+      const int nx = (l + 1) & 63;
+      tgr_emu::tl_lane = nx;
+      setcontext(&r->wave.ctx[nx]);
+    }
+  }
+};
+
+template <int NEQ>
+void run_jac(const tgr_emu_jac_args& a, const tgr::DevMech& m) {
+  JacRun<NEQ>* r = new JacRun<NEQ>();
+  r->a = &a;
+  r->m = m;
+  r->ws.assign((size_t)NEQ * tgr::kWave, 0.0);
+  const size_t stack = 1 << 20;
+  std::vector<char> stacks(stack * 64);
+  tgr_emu::tw = &r->wave;
+  for (int l = 0; l < 64; ++l) {
+    getcontext(&r->wave.ctx[l]);
+    r->wave.ctx[l].uc_stack.ss_sp = stacks.data() + stack * l;
+    r->wave.ctx[l].uc_stack.ss_size = stack;
+    r->wave.ctx[l].uc_link = nullptr;
+    const uintptr_t u = (uintptr_t)r;
+    makecontext(&r->wave.ctx[l], (void (*)())JacRun<NEQ>::lane_main, 3, (unsigned)(u >> 32),
+                (unsigned)(u & 0xffffffffu), l);
+  }
+  tgr_emu::tl_lane = 0;
+  swapcontext(&r->wave.home, &r->wave.ctx[0]);
+  delete r;
+}
+
 }  // namespace
+
+// Both Jacobians (dq_jac, jac_split) on n packed states sol[n][neq]
+// ([rho Y_k, rho T]) with step size h for the increments: jdq, jsplit
+// [n][neq][neq] (row-major), f[n][neq] the difference of the two base
+// right-hand sides (zero when the fixed-temperature evaluation reproduces
+// the plain one).
+extern "C" int emu_jac(const char* mech, long n, double p_gp, double rtol, double atol, double h, const double* sol,
+                       double* jdq, double* jsplit, double* f) {
+  const tgr_mech_tables* t = find_mech(mech);
+  if (!t) return -1;
+  std::vector<char> blob;
+  tgr_mech_info info;
+  std::string err;
+  if (tgr::build_table_blob(t, blob, &info, &err)) return -2;
+  tgr::DevMech m;
+  m.tab = blob.data();
+  m.nsp = t->nsp;
+  m.tmid = t->tmid;
+  tgr_emu_jac_args a{n, p_gp, rtol, atol, h, sol, jdq, jsplit, f};
+  switch (t->nsp + 1) {
+    case 10: run_jac<10>(a, m); break;
+    case 37: run_jac<37>(a, m); break;
+    case 40: run_jac<40>(a, m); break;
+    case 46: run_jac<46>(a, m); break;
+    default: return -3;
+  }
+  return 0;
+}
 
 // One chemistry step over ncell packed cells: rho[ncell], sc[(nsp+1) * ncell]
 // (mass fractions then T, overwritten), hr[ncell], chemsrc[3 * ncell],

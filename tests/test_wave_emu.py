@@ -24,6 +24,8 @@ def emu():
     lib.emu_step.argtypes = [ctypes.c_char_p, ctypes.c_long, vp, vp, ctypes.c_double, ctypes.c_double,
                              ctypes.c_double, ctypes.c_double, ctypes.c_int, vp, vp, vp]
     lib.emu_lane_rhs.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_double, vp, vp]
+    lib.emu_jac.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                            ctypes.c_double, vp, vp, vp, vp]
     return lib
 
 
@@ -118,3 +120,47 @@ def test_emulated_failure_paths_match_oracle(emu, oracle):
     for c in (0, 3):
         assert (cnt[c, :9] == st[c, :9]).all()
         assert np.abs(sc[-1, c] - ref[c, -1] / rho[c]) < 1e-6
+
+
+@pytest.mark.parametrize("workload", ["h2", "gri30", "ethyred", "dme39"])
+def test_split_jacobian_against_central_differences(emu, workload):
+    """Bdf::jac_split (species columns at fixed T reusing the rate constants,
+    plus df/dT times dT/dy) against a central-difference Jacobian of the
+    generated lane right-hand side with increments far above rounding: it is
+    at least as close as the plain difference-quotient Jacobian (Bdf::dq_jac,
+    whose increments for trace species are below the rounding of rho, so it
+    misses their coupling through T = rho T / rho), and its base evaluation is
+    the plain right-hand side bit for bit."""
+    _, Y, T, rho, P, dt = inputs.states(workload, n=4, seed=3)
+    sol = np.ascontiguousarray(inputs.packed_state(Y, T, rho))
+    n, neq = sol.shape
+    jd = np.zeros((n, neq, neq))
+    js = np.zeros_like(jd)
+    df = np.zeros((n, neq))
+    assert emu.emu_jac(workload.encode(), n, P, 1e-6, 1e-12, dt / 255, sol.ctypes.data, jd.ctypes.data,
+                       js.ctypes.data, df.ctypes.data) == 0
+    assert np.all(df == 0.0)
+
+    def rhs(s):
+        out = np.zeros_like(s)
+        s = np.ascontiguousarray(s)
+        assert emu.emu_lane_rhs(workload.encode(), n, P, s.ctypes.data, out.ctypes.data) == 0
+        return out
+    jr = np.zeros_like(jd)
+    r = sol[:, :neq - 1].sum(axis=1)
+    for j in range(neq):
+        hj = 1e-6 * np.maximum(np.abs(sol[:, j]), 1e-4 * (r if j < neq - 1 else sol[:, j]))
+        sp, sm = sol.copy(), sol.copy()
+        sp[:, j] += hj
+        sm[:, j] -= hj
+        jr[:, :, j] = (rhs(sp) - rhs(sm)) / (2.0 * hj[:, None])
+    nr = np.linalg.norm(jr.reshape(n, -1), axis=1)
+    es = np.linalg.norm((js - jr).reshape(n, -1), axis=1) / nr
+    ed = np.linalg.norm((jd - jr).reshape(n, -1), axis=1) / nr
+    print("\n%s: relative Frobenius distance to central differences: split %.2e, plain %.2e"
+          % (workload, es.max(), ed.max()))
+    assert es.max() <= 5e-3 and np.all(es <= 1.5 * ed)
+    # the y_T column is the same difference quotient in both (up to the
+    # increment: rho T vs T)
+    tc = np.abs(js[:, :, -1] - jd[:, :, -1]).max() / np.abs(jd[:, :, -1]).max()
+    assert tc <= 1e-5, tc

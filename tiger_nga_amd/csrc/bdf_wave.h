@@ -80,6 +80,9 @@ struct Bdf {
   TGR_LDS double* tau;
   int q, qprime, qwait, Lq, jcur, nst, nstlp, nstlj;
   int c_nfe, c_nfe_dq, c_nje, c_nsetups, c_nni, c_netf, c_ncfn, c_nlu;
+  // one-step SDIRK solves: rate cache for the split Jacobian (nullptr: the
+  // plain difference-quotient Jacobian)
+  TGR_LDS RateCache<NEQ>* rc = nullptr;
 
   __device__ Bdf(const DevMech& mech, tchar* tabl, TGR_LDS RhsLds<NEQ>* rl, TGR_LDS double (&znl)[kQmaxAbs + 2][kWave],
                  double* ws, double pgp, double rt, double at, int qm, int ms)
@@ -189,6 +192,42 @@ struct Bdf {
       const double ii = 1.0 / inc;
       fresh(jw)[j * kWave + lane] = ii * fp - ii * fy;
     }
+  }
+
+  // The Jacobian with the temperature-dependent factors evaluated once: with
+  // T = y_T / rho, J(:, j) = df/dy_j|_T + df/dT|_y dT/dy_j, dT/dy_j = -T / rho
+  // for a species and 1 / rho for y_T.  The species columns are difference
+  // quotients at fixed T (increments as in dq_jac) that reuse the base
+  // evaluation's rate constants and thermo (rc); df/dT is one difference
+  // quotient in T.  Stores J in jw and returns f(yv).
+  __device__ __forceinline__ double jac_split(double yv) {
+    constexpr int nsp = NEQ - 1;
+    const double y0 = act ? yv : 0.0;
+    const double rho = wave_sum(lane < nsp ? y0 : 0.0);
+    const double T = bcast(y0, nsp) / rho;
+    const double fy = rhs_call_T<NEQ, 1>(tab, S, rc, m.tmid, y0, p_gp, T);
+    c_nfe++;
+    const double srur = sqrt(UROUND);
+    const double fnorm = wrms(fy);
+    const double minInc = (fnorm != 0.0) ? (MIN_INC_MULT * fabs(h) * UROUND * n * fnorm) : 1.0;
+    for (int j = 0; j < nsp; ++j) {
+      const double yj = bcast(y0, j);
+      const double ewj = bcast(ewt, j);
+      const double inc = fmax(srur * fabs(yj), minInc / ewj);
+      const double yp = (lane == j) ? yj + inc : y0;
+      const double fp = rhs_call_T<NEQ, 2>(tab, S, rc, m.tmid, yp, p_gp, T);
+      c_nfe_dq++;
+      const double ii = 1.0 / inc;
+      fresh(jw)[j * kWave + lane] = ii * fp - ii * fy;
+    }
+    const double dT = srur * T;
+    const double fT = rhs_call_T<NEQ, 0>(tab, S, rc, m.tmid, y0, p_gp, T + dT);
+    c_nfe_dq++;
+    const double dfdT = act ? (fT - fy) / dT : 0.0;
+    const double sy = -T / rho;
+    for (int j = 0; j < nsp; ++j) fresh(jw)[j * kWave + lane] += sy * dfdT;
+    fresh(jw)[nsp * kWave + lane] = dfdT / rho;
+    return fy;
   }
 
   __device__ __forceinline__ int lsetup(int convfail) {
@@ -745,8 +784,14 @@ struct Bdf {
     nst = 0;
     const double y0 = act ? yv : 0.0;
     set_ewt(y0);
-    const double f0 = f(y0);
-    c_nfe++;
+    h = h0;
+    double f0;
+    if (rc) {
+      f0 = jac_split(y0);
+    } else {
+      f0 = f(y0);
+      c_nfe++;
+    }
     if (h0_fac > 0.0) {
       // first step from the initial rate: the step that moves the state by
       // h0_fac acceptance tolerances at that rate
@@ -754,8 +799,10 @@ struct Bdf {
       h0 = (r0 > 0.0) ? h0_fac / r0 : dt;
       h0 = fmin(dt, fmax(dt * (1.0 / 1024.0), h0));
     }
-    h = h0;
-    dq_jac(y0, f0);
+    if (!rc) {
+      h = h0;
+      dq_jac(y0, f0);
+    }
     c_nje++;
     double yi = y0, t = 0.0, hh = fmin(h0, dt);
     double kl = act ? f0 : 0.0;  // derivative at yi (the last stage's, the method being stiffly accurate)
@@ -861,9 +908,13 @@ struct Bdf {
     h = hs[0];
     const double y0 = act ? yv : 0.0;
     set_ewt(y0);
-    const double f0 = f(y0);
-    c_nfe++;
-    dq_jac(y0, f0);
+    if (rc) {
+      jac_split(y0);
+    } else {
+      const double f0 = f(y0);
+      c_nfe++;
+      dq_jac(y0, f0);
+    }
     c_nje++;
     double yi = y0;
     // Newton starts from the previous stage's converged value plus the

@@ -105,7 +105,7 @@ struct tgr_chem {
   // below surr_min_accept, calls go to the plain step, with a guided probe
   // every surr_probe calls (0: always guided)
   int surr_probe = 8, surr_skipped = 0;
-  int surr_adaptive = 1, surr_max_att = 48;
+  int surr_adaptive = 1, surr_max_att = 48, surr_split_jac = 1;
   double surr_h0_fac = 0.0, surr_grow = 10.0, surr_keep = 1.5;
   double surr_min_accept = 0.15, surr_last_accept = 1.0;
   double surr_newton_tol = 0.1, surr_tol_y = 2.5e-7, surr_tol_t = 1.0e-2, surr_grade = 2.0;
@@ -383,6 +383,9 @@ int tgr_chem_set_option(tgr_chem* c, const char* key, int64_t v) {
   } else if (!strcmp(key, "surr_guess")) {
     if (v != 0 && v != 1) return fail(TGR_E_ARG, "surr_guess must be 0 or 1");
     c->surr_guess = (int)v;
+  } else if (!strcmp(key, "surr_split_jac")) {
+    if (v != 0 && v != 1) return fail(TGR_E_ARG, "surr_split_jac must be 0 or 1");
+    c->surr_split_jac = (int)v;
   } else if (!strcmp(key, "surr_adaptive")) {
     if (v != 0 && v != 1) return fail(TGR_E_ARG, "surr_adaptive must be 0 or 1");
     c->surr_adaptive = (int)v;
@@ -584,6 +587,7 @@ int run_surrogate(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, 
   P.h0_fac = c->surr_h0_fac;
   P.ntau = nst;
   P.grow = c->surr_grow;
+  P.split_jac = c->surr_split_jac;
   P.keep = c->surr_keep;
   {
     // times of the network outputs, in the order they were computed

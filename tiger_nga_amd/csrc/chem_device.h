@@ -242,16 +242,32 @@ __device__ __forceinline__ double troe(double kinf, double k0, double conc, doub
   return kinf * F * pr / (1.0 + pr);
 }
 
+// Temperature-dependent factors of one right-hand side, kept for others at
+// the same temperature (the species columns of a difference-quotient
+// Jacobian taken at fixed T): rate constants of every reaction lane (the
+// high-pressure limits on fall-off lanes), the low-pressure limits and
+// ln Fcent of the fall-off lanes, and each species' h and cp.  Every lane
+// reads back only what it wrote.
+template <int NEQ>
+struct RateCache {
+  double kf[((MechT<NEQ>::nrxn + 63) / 64) * 64];
+  double kb[((MechT<NEQ>::nrxn + 63) / 64) * 64];
+  double k0f[kWave], k0b[kWave], lnfc[kWave];
+  double h[kWave], cp[kWave];
+};
+
 // Production rates CDOT (kmol/m3/s) of this lane's species from the
 // concentrations (lane k holds C_k), temperature and pressure.  ctot is the
-// total concentration sum_k C_k (wave-uniform).
-template <int NEQ>
+// total concentration sum_k C_k (wave-uniform).  MODE 1 also saves the
+// temperature-dependent factors into rc; MODE 2 takes them from rc instead of
+// evaluating them (T must be the temperature they were saved at).
+template <int NEQ, int MODE = 0>
 __device__ __forceinline__ double prodrates_wave(tchar* tab, TGR_LDS RhsLds<NEQ>& S, double Ck, double ctot,
-                                                 double T, double P) {
+                                                 double T, double P, TGR_LDS RateCache<NEQ>* rc = nullptr) {
   using MT = MechT<NEQ>;
   constexpr TabLayout L = tab_layout<NEQ>();
   const int lane = lane_id();
-  const double lt = flog(T);
+  const double lt = MODE == 2 ? 0.0 : flog(T);
   const double rt = kRgasF4 * T;
   const double irt = 1.0 / rt;
   S.C[lane] = lane == kOne ? 1.0 : Ck;
@@ -271,29 +287,52 @@ __device__ __forceinline__ double prodrates_wave(tchar* tab, TGR_LDS RhsLds<NEQ>
     tchar* rx = tab + L.rx + rd * kRxRound + lane * 8;
     tchar* rxi = tab + L.rx + rd * kRxRound + 6 * kRow + lane * 4;
     const int i0 = ld<int>(rxi, 0), i1 = ld<int>(rxi, kIRow);
-    double kf = arrhenius(ld<double>(rx, 0), ld<double>(rx, kRow), ld<double>(rx, 2 * kRow), lt, irt);
-    double kb = arrhenius(ld<double>(rx, 3 * kRow), ld<double>(rx, 4 * kRow), ld<double>(rx, 5 * kRow), lt, irt);
+    double kf, kb;
+    if (MODE == 2) {
+      kf = rc->kf[rd * 64 + lane];
+      kb = rc->kb[rd * 64 + lane];
+    } else {
+      kf = arrhenius(ld<double>(rx, 0), ld<double>(rx, kRow), ld<double>(rx, 2 * kRow), lt, irt);
+      kb = arrhenius(ld<double>(rx, 3 * kRow), ld<double>(rx, 4 * kRow), ld<double>(rx, 5 * kRow), lt, irt);
+      if (MODE == 1) {
+        rc->kf[rd * 64 + lane] = kf;
+        rc->kb[rd * 64 + lane] = kb;
+      }
+    }
     if (rd == 0 && ((i1 >> 24) & 1)) {
       tchar* fo = tab + L.fo + lane * 8;
       const int fi = ld<int>(tab, L.fo + 12 * kRow + lane * 4);
-      const double invT = 1.0 / T;
-      double fc = 0.0;
+      double lnfc, k0f, k0b;
+      if (MODE == 2) {
+        lnfc = rc->lnfc[lane];
+        k0f = rc->k0f[lane];
+        k0b = rc->k0b[lane];
+      } else {
+        const double invT = 1.0 / T;
+        double fc = 0.0;
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        const double x = -(ld<double>(fo, (9 + i) * kRow) * (((fi >> i) & 1) ? invT : T));
-        fc = fma(ld<double>(fo, (6 + i) * kRow), fexp(x), fc);
+        for (int i = 0; i < 3; ++i) {
+          const double x = -(ld<double>(fo, (9 + i) * kRow) * (((fi >> i) & 1) ? invT : T));
+          fc = fma(ld<double>(fo, (6 + i) * kRow), fexp(x), fc);
+        }
+        lnfc = flog(fc);
+        k0f = arrhenius(ld<double>(fo, 0), ld<double>(fo, kRow), ld<double>(fo, 2 * kRow), lt, irt);
+        k0b = ((i1 >> 25) & 1)
+                  ? arrhenius(ld<double>(fo, 3 * kRow), ld<double>(fo, 4 * kRow), ld<double>(fo, 5 * kRow), lt, irt)
+                  : 0.0;
+        if (MODE == 1) {
+          rc->lnfc[lane] = lnfc;
+          rc->k0f[lane] = k0f;
+          rc->k0b[lane] = k0b;
+        }
       }
       const double mv = S.M[(fi >> 8) & 255];
       const double conc = mv > 0.0 ? mv : P / rt;
-      const double lnfc = flog(fc);
       const double lfc = lnfc * kInvLn10;
       const double ntmp = fma(-kTroeN1, lfc, kTroeN0);
       const double cco = fma(-kTroeC1, lfc, -kTroeC0);
-      kf = troe(kf, arrhenius(ld<double>(fo, 0), ld<double>(fo, kRow), ld<double>(fo, 2 * kRow), lt, irt), conc,
-                lnfc, ntmp, cco);
-      if ((i1 >> 25) & 1)
-        kb = troe(kb, arrhenius(ld<double>(fo, 3 * kRow), ld<double>(fo, 4 * kRow), ld<double>(fo, 5 * kRow), lt, irt),
-                  conc, lnfc, ntmp, cco);
+      kf = troe(kf, k0f, conc, lnfc, ntmp, cco);
+      if ((i1 >> 25) & 1) kb = troe(kb, k0b, conc, lnfc, ntmp, cco);
     }
     const double wf = kf * S.C[i0 & 255] * S.C[(i0 >> 8) & 255] * S.C[(i0 >> 16) & 255];
     const double wb = kb * S.C[i1 & 255] * S.C[(i1 >> 8) & 255] * S.C[(i1 >> 16) & 255];
@@ -338,6 +377,42 @@ __device__ __forceinline__ double rhs_wave(tchar* tab, TGR_LDS RhsLds<NEQ>& S, d
   const double rk = sp ? cdot * ld<double>(tab, L.w + lane * 8) : 0.0;
   const double tmp = wave_sum(-h * rk);
   return sp ? rk : (lane == nsp ? tmp / cpm : 0.0);
+}
+
+// The right-hand side with the temperature given (instead of y[nsp] / rho);
+// MODE as in prodrates_wave, MODE 2 also takes h and cp from rc.
+template <int NEQ, int MODE>
+__device__ __forceinline__ double rhs_wave_T(tchar* tab, TGR_LDS RhsLds<NEQ>& S, TGR_LDS RateCache<NEQ>* rc,
+                                             double tmid, double y, double p_gp, double T) {
+  constexpr int nsp = NEQ - 1;
+  constexpr TabLayout L = tab_layout<NEQ>();
+  const int lane = lane_id();
+  const bool sp = lane < nsp;
+  const double rho = wave_sum(sp ? y : 0.0);
+  double h, cp;
+  if (MODE == 2) {
+    h = rc->h[lane];
+    cp = rc->cp[lane];
+  } else {
+    thermo_lane<NEQ>(tab, tmid, T, lane, h, cp);
+    if (MODE == 1) {
+      rc->h[lane] = h;
+      rc->cp[lane] = cp;
+    }
+  }
+  const double Ck = sp ? y * ld<double>(tab, L.iw + lane * 8) : 0.0;
+  const double cpm = wave_sum(sp ? (y / rho) * cp : 0.0);
+  const double ctot = wave_sum(Ck);
+  const double cdot = prodrates_wave<NEQ, MODE>(tab, S, Ck, ctot, T, p_gp, rc);
+  const double rk = sp ? cdot * ld<double>(tab, L.w + lane * 8) : 0.0;
+  const double tmp = wave_sum(-h * rk);
+  return sp ? rk : (lane == nsp ? tmp / cpm : 0.0);
+}
+
+template <int NEQ, int MODE>
+__device__ __attribute__((noinline)) double rhs_call_T(tchar* tab, TGR_LDS RhsLds<NEQ>* S, TGR_LDS RateCache<NEQ>* rc,
+                                                       double tmid, double y, double p_gp, double T) {
+  return rhs_wave_T<NEQ, MODE>(tab, *S, rc, tmid, y, p_gp, T);
 }
 
 // Out-of-line right-hand side for the integrator: one copy of the code

@@ -65,6 +65,7 @@ struct SurrParams {
   int max_att;                   // adaptive: attempted steps before the cell is rejected
   double h0_fac;                 // adaptive: first step c / max_k |f0_k| wacc_k in [dt/1024, dt] (0: hsub[0])
   int ntau;                      // number of network output times
+  int split_jac;                 // 1: Jacobian with the rate constants evaluated once (Bdf::jac_split)
   double grow;                   // adaptive: largest step growth factor
   double keep;                   // adaptive: growth factors in [1, keep) keep h (and the LU factors)
   double tau[kSurrStages * kSurrMaxSub];  // times of the network outputs, increasing

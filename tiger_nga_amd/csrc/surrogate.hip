@@ -88,7 +88,7 @@ __global__ void surr_reset_kernel(unsigned int* queue, unsigned int* rej_count, 
 // rejected (Newton, error estimate, enthalpy check; fields untouched).
 template <int NEQ>
 __device__ __forceinline__ int surr_cell(const SurrParams& P, const DevMech& m, tchar* tab, TGR_LDS RhsLds<NEQ>* rl,
-                                         int64_t cell, double* ws,
+                                         TGR_LDS RateCache<NEQ>* rc, int64_t cell, double* ws,
                                          int& nit, int& nfe, int& nlu) {
   const StepParams& p = P.s;
   const int lane = lane_id();
@@ -112,6 +112,7 @@ __device__ __forceinline__ int surr_cell(const SurrParams& P, const DevMech& m, 
   };
   const double wacc = lane < nsp ? 1.0 / (P.tol_y * rho) : (lane == nsp ? 1.0 / (P.tol_t * rho) : 0.0);
   Bdf<NEQ> b(m, tab, rl, ws, p.p_gp, p.rtol, p.atol);
+  if (P.split_jac) b.rc = rc;
   int st;
   if (P.adaptive) {
     // the network's state at time t: linear in t between its stage times
@@ -236,6 +237,7 @@ __global__ __launch_bounds__(kSurrBlock) void surr_kernel(SurrParams P) {
   constexpr TabLayout L = tab_layout<NEQ>();
   __shared__ __attribute__((aligned(16))) char tab_s[L.bytes];
   __shared__ RhsLds<NEQ> rl[kSurrWaves];
+  __shared__ RateCache<NEQ> rcs[kSurrWaves];
   stage_tables((TGR_LDS char*)tab_s, P.s.m.tab, L.bytes);
   __syncthreads();
   const DevMech m = P.s.m;
@@ -248,8 +250,8 @@ __global__ __launch_bounds__(kSurrBlock) void surr_kernel(SurrParams P) {
     const int64_t cell = (int64_t)(unsigned int)__builtin_amdgcn_readfirstlane((int)c);
     if (cell >= P.s.ncell) break;
     int nit = 0, nfe = 0, nlu = 0;
-    const int st = surr_cell<NEQ>(P, m, (tchar*)tab_s, (TGR_LDS RhsLds<NEQ>*)&rl[w], cell,
-                                  ws, nit, nfe, nlu);
+    const int st = surr_cell<NEQ>(P, m, (tchar*)tab_s, (TGR_LDS RhsLds<NEQ>*)&rl[w],
+                                  (TGR_LDS RateCache<NEQ>*)&rcs[w], cell, ws, nit, nfe, nlu);
     if (st < 0) continue;
     if (lane == 0) {
       if (st != 0) {

@@ -42,6 +42,17 @@ def rhs_flops(info: dict) -> float:
     return f
 
 
+def rhs_flops_cached(info: dict) -> float:
+    """One right-hand side at a temperature whose factors are already known
+    (the species columns of the guided solve's split Jacobian): no ln T,
+    thermo polynomials, Arrhenius terms or Troe centres; the fall-off
+    blending (Pr, F) is still evaluated."""
+    nsp = info["nsp"]
+    return (rhs_flops(info) - nsp * 20 - LOG - ARRH * (info["nrxn"] + info["n_reversible"])
+            - info["n_fc_terms"] * (1 + EXP + 2) - info["n_falloff"] * LOG
+            - ARRH * (info["n_falloff"] + info["n_rev_falloff"]))
+
+
 def step_flops(info: dict, stats: dict) -> float:
     """Total algorithmic flops of one call from its integrator counters."""
     n = info["nsp"] + 1
@@ -68,18 +79,22 @@ def properties_flops(nsp: int) -> float:
     return 3 * nsp + 19 * nsp + 10 * nsp * nsp + 12 * nsp + 5 + nsp
 
 
-def guided_flops(info: dict, sstats: dict, nsub: int = 0) -> float:
+def guided_flops(info: dict, sstats: dict, nsub: int = 0, split: bool = False) -> float:
     """Algorithmic flops of the surrogate step's guided solve (before the plain
     fallback) from its counters: every right-hand side (Newton and
     difference-quotient columns), one LU of I - gamma h J and one filtered
     error estimate per attempted sub-step (the counted ones; without that
     counter, nsub per attempted cell), a triangular solve and update per
     Newton iteration, and the enthalpy check / element projection (taken as
-    60 neq per cell)."""
+    60 neq per cell).  split: the Jacobian's nsp species columns per cell are
+    fixed-temperature evaluations (rhs_flops_cached)."""
     n = info["nsp"] + 1
     cells = sstats["accepted"] + sstats["rejected"]
     f_lu = 2.0 * n * n + 2.0 / 3.0 * n ** 3
     f_solve = 2.0 * n * n + 4.0 * n
     subs = sstats.get("substeps") or cells * nsub
-    return (sstats["rhs_evals"] * rhs_flops(info) + subs * (f_lu + f_solve)
-            + sstats["newton_iters"] * f_solve + cells * 60.0 * n)
+    f = (sstats["rhs_evals"] * rhs_flops(info) + subs * (f_lu + f_solve)
+         + sstats["newton_iters"] * f_solve + cells * 60.0 * n)
+    if split:
+        f -= cells * info["nsp"] * (rhs_flops(info) - rhs_flops_cached(info))
+    return f
