@@ -9,7 +9,8 @@ Choices (all exposed as context options of include/tiger_chem.h):
   capacity (occupancy x CUs) or a fraction of it (fewer workgroups leave CUs
   idle but reduce contention for the L2 and the work counter at small sizes);
 * the surrogate-guided solve (when a trained network exists): schedule
-  (``surr_adaptive``, ``surr_nsub``, ``surr_grade``, ``surr_max_att``),
+  (``surr_adaptive``, ``surr_nsub``, ``surr_grade``, ``surr_max_att``,
+  ``surr_h0_fac``),
   network use (``surr_guess``) and acceptance threshold (``surr_tol_y``
   2.5e-7 .. 2e-6): the fastest combination, timed on freshly generated plus relaxed
   states (the block after one plain step), whose max species error against
@@ -39,6 +40,8 @@ SURR_SCHEDULES = ((1, 8, 2.0, 48), (1, 4, 1.0, 32), (1, 8, 3.0, 64), (0, 4, 1.0,
 # grows with the number of steps it takes (each within the threshold), so the
 # mechanisms that take many need the smaller ones
 SURR_TOLS = (2.5e-7, 5e-7, 1e-6, 2e-6)
+# first step of the adaptive schedule from the initial rate (0: the schedule's)
+SURR_H0 = (0.0, 4.0, 16.0)
 
 
 def _device_name():
@@ -129,38 +132,46 @@ def tune(mech: str, cells: int = 32 ** 3, seed: int = 7, reps: int = 2, log=prin
         fc.set_option("surr_probe", 0)
         st_all = {}
         best = None
+        combos = []
         for (adaptive, nsub, grade, att) in SURR_SCHEDULES:
             for guess in (0, 1):
-                for tol in SURR_TOLS:
-                    fc.set_option("surr_adaptive", adaptive)
-                    fc.set_option("surr_nsub", nsub)
-                    fc.set_option("surr_grade", grade)
-                    fc.set_option("surr_max_att", att)
-                    fc.set_option("surr_guess", guess)
-                    fc.set_option("surr_tol_y", tol)
-                    t_tot, errs = 0.0, []
-                    for (Rb, Sb0, gb, Pb, ref, _) in sets:
-                        Sb = Sb0.clone()
+                # the rate-based first step only for the adaptive schedule
+                # without the network (the network's first-step guess assumes
+                # the schedule's first sub-step)
+                for h0f in (SURR_H0 if adaptive and not guess else (0.0,)):
+                    combos.append((adaptive, nsub, grade, att, guess, h0f))
+        for (adaptive, nsub, grade, att, guess, h0f) in combos:
+            for tol in SURR_TOLS:
+                fc.set_option("surr_adaptive", adaptive)
+                fc.set_option("surr_nsub", nsub)
+                fc.set_option("surr_grade", grade)
+                fc.set_option("surr_max_att", att)
+                fc.set_option("surr_guess", guess)
+                fc.set_option("surr_h0_fac", h0f)
+                fc.set_option("surr_tol_y", tol)
+                t_tot, errs = 0.0, []
+                for (Rb, Sb0, gb, Pb, ref, _) in sets:
+                    Sb = Sb0.clone()
 
-                        def sstep(Rb=Rb, Sb=Sb, Sb0=Sb0, gb=gb, Pb=Pb):
-                            Sb.copy_(Sb0)
-                            fc.source_chemistry_surrogate(Rb, Sb, gb, dt, Pb, Pb, sync=False)
-                        t_tot += _time(torch, sstep, reps)
-                        sstep()
-                        torch.cuda.synchronize()
-                        errs.append(float((Sb[..., :nsp] - ref[..., :nsp]).abs().max()))
-                    name = "%s n%d r%.1f a%d guess%d tol%.1e" % ("adaptive" if adaptive else "fixed", nsub, grade, att,
-                                                               guess, tol)
-                    st_all[name] = {"ms_generated_plus_relaxed": t_tot * 1e3, "max_species_err_generated": errs[0],
-                                    "max_species_err_relaxed": errs[1]}
-                    log("  surrogate %s: %.2f ms, max |dY| %.2e / %.2e" % (name, t_tot * 1e3, errs[0], errs[1]))
-                    # half the 1e-6 bound: larger blocks reach further into the tail
-                    if max(errs) <= 5e-7 and (best is None or t_tot < best[0]):
-                        best = (t_tot, adaptive, nsub, grade, att, guess, tol)
+                    def sstep(Rb=Rb, Sb=Sb, Sb0=Sb0, gb=gb, Pb=Pb):
+                        Sb.copy_(Sb0)
+                        fc.source_chemistry_surrogate(Rb, Sb, gb, dt, Pb, Pb, sync=False)
+                    t_tot += _time(torch, sstep, reps)
+                    sstep()
+                    torch.cuda.synchronize()
+                    errs.append(float((Sb[..., :nsp] - ref[..., :nsp]).abs().max()))
+                name = "%s n%d r%.1f a%d guess%d h0f%g tol%.1e" % ("adaptive" if adaptive else "fixed", nsub,
+                                                                 grade, att, guess, h0f, tol)
+                st_all[name] = {"ms_generated_plus_relaxed": t_tot * 1e3, "max_species_err_generated": errs[0],
+                                "max_species_err_relaxed": errs[1]}
+                log("  surrogate %s: %.2f ms, max |dY| %.2e / %.2e" % (name, t_tot * 1e3, errs[0], errs[1]))
+                # half the 1e-6 bound: larger blocks reach further into the tail
+                if max(errs) <= 5e-7 and (best is None or t_tot < best[0]):
+                    best = (t_tot, adaptive, nsub, grade, att, guess, h0f, tol)
         rec["measured"]["surrogate"] = st_all
         if best:
             (_, rec["surr_adaptive"], rec["surr_nsub"], rec["surr_grade"], rec["surr_max_att"], rec["surr_guess"],
-             rec["surr_tol_y"]) = best
+             rec["surr_h0_fac"], rec["surr_tol_y"]) = best
             rec["measured"]["surrogate_speedup"] = (sets[0][5] + sets[1][5]) / best[0]
         fc.set_surrogate(None, None, None, None)
     fc.close()
@@ -198,7 +209,7 @@ def apply(fc, cells: int, path=CACHE) -> dict:
     if not rec:
         return done
     for k in ("rhs_mapping", "grid", "surr_adaptive", "surr_nsub", "surr_grade", "surr_max_att", "surr_guess",
-              "surr_tol_y"):
+              "surr_h0_fac", "surr_tol_y"):
         if k in rec:
             fc.set_option(k, rec[k])
             done[k] = rec[k]
