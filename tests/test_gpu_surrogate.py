@@ -27,19 +27,24 @@ def _model(mech):
     return surrogate.Surrogate.load(path)
 
 
-@pytest.mark.parametrize("mech,workload,n,relax,adaptive,guess", [
-    ("h2", "h2", 16 ** 3, 0, 1, 0), ("h2", "h2", 16 ** 3, 1, 1, 0), ("gri30", "gri30", 16 ** 3, 0, 1, 0),
-    ("gri30", "gri30", 16 ** 3, 1, 1, 0), ("gri30", "gri30", 16 ** 3, 0, 1, 1), ("gri30", "gri30", 16 ** 3, 1, 0, 1),
-    ("ethyred", "ethyred", 16 ** 3, 0, 1, 0)])
-def test_cells_accepted_within_bound_or_plain(torch_dev, mech, workload, n, relax, adaptive, guess):
+@pytest.mark.parametrize("mech,workload,n,relax,adaptive,guess,split,h0f", [
+    ("h2", "h2", 16 ** 3, 0, 1, 0, 1, 0.0), ("h2", "h2", 16 ** 3, 1, 1, 0, 1, 0.0),
+    ("gri30", "gri30", 16 ** 3, 0, 1, 0, 1, 0.0), ("gri30", "gri30", 16 ** 3, 1, 1, 0, 1, 0.0),
+    ("gri30", "gri30", 16 ** 3, 0, 1, 1, 1, 0.0), ("gri30", "gri30", 16 ** 3, 1, 0, 1, 1, 0.0),
+    ("gri30", "gri30", 16 ** 3, 0, 1, 0, 0, 0.0), ("gri30", "gri30", 16 ** 3, 0, 1, 0, 1, 16.0),
+    ("gri30", "gri30", 16 ** 3, 1, 1, 0, 1, 16.0), ("ethyred", "ethyred", 16 ** 3, 0, 1, 0, 1, 0.0),
+    ("dme39", "dme39", 16 ** 3, 0, 1, 0, 1, 0.0)])
+def test_cells_accepted_within_bound_or_plain(torch_dev, mech, workload, n, relax, adaptive, guess, split, h0f):
     """adaptive: error-controlled (1) or fixed (0) schedule; guess: Newton
-    from the network's prediction (1) or without the network (0)."""
+    from the network's prediction (1) or without the network (0); split:
+    the Jacobian with the rate constants evaluated once (1) or every column a
+    full difference quotient (0); h0f: first step from the initial rate."""
     torch = torch_dev
     from tiger_nga_amd import surrogate
     from tiger_nga_amd.finitechem import FiniteChem
     fc = FiniteChem(mech, 1e-6, 1e-12)
     R_d, S0, g, P = surrogate.block_states(fc, workload, n, 3, relax)
-    dt = {"h2": 1e-6, "gri30": 5e-7, "ethyred": 5e-7}[mech]
+    dt = {"h2": 1e-6, "gri30": 5e-7, "ethyred": 5e-7, "dme39": 2e-7}[mech]
     nsp = fc.nsp
     Sp = S0.clone()
     HRp = torch.zeros_like(R_d)
@@ -54,6 +59,8 @@ def test_cells_accepted_within_bound_or_plain(torch_dev, mech, workload, n, rela
     sur.attach(fc)
     fc.set_option("surr_adaptive", adaptive)
     fc.set_option("surr_guess", guess)
+    fc.set_option("surr_split_jac", split)
+    fc.set_option("surr_h0_fac", h0f)
     if not adaptive:
         fc.set_option("surr_nsub", 4)
         fc.set_option("surr_grade", 1.0)
@@ -82,8 +89,9 @@ def test_cells_accepted_within_bound_or_plain(torch_dev, mech, workload, n, rela
     cons_plain = float(((elp - el0).abs() / el0.abs().clamp_min(1e-30)).max())
     print("\nelement conservation: guided %.2e, plain %.2e (max relative change)" % (cons, cons_plain))
     assert cons <= max(1e-12, 2.0 * cons_plain)
-    print("\n%s relax %d adaptive %d guess %d: accepted %d / %d, rejected (Newton %d, error %d, enthalpy %d); "
-          "max|dY| %.2e max|dT| %.2e K" % (mech, relax, adaptive, guess, ss["accepted"], n, ss["rej_newton"],
+    print("\n%s relax %d adaptive %d guess %d split %d h0f %g: accepted %d / %d, rejected (Newton %d, error %d, "
+          "enthalpy %d); max|dY| %.2e max|dT| %.2e K" % (mech, relax, adaptive, guess, split, h0f, ss["accepted"], n,
+                                                         ss["rej_newton"],
                                            ss["rej_error"], ss["rej_enthalpy"], float(dY.max()), float(dT.max())))
     assert ss["substeps"] >= ss["accepted"]
     if adaptive and relax:
