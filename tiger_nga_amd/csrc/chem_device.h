@@ -288,13 +288,13 @@ __device__ __forceinline__ double prodrates_wave(tchar* tab, TGR_LDS RhsLds<NEQ>
     tchar* rxi = tab + L.rx + rd * kRxRound + 6 * kRow + lane * 4;
     const int i0 = ld<int>(rxi, 0), i1 = ld<int>(rxi, kIRow);
     double kf, kb;
-    if (MODE == 2) {
+    if constexpr (MODE == 2) {
       kf = rc->kf[rd * 64 + lane];
       kb = rc->kb[rd * 64 + lane];
     } else {
       kf = arrhenius(ld<double>(rx, 0), ld<double>(rx, kRow), ld<double>(rx, 2 * kRow), lt, irt);
       kb = arrhenius(ld<double>(rx, 3 * kRow), ld<double>(rx, 4 * kRow), ld<double>(rx, 5 * kRow), lt, irt);
-      if (MODE == 1) {
+      if constexpr (MODE == 1) {
         rc->kf[rd * 64 + lane] = kf;
         rc->kb[rd * 64 + lane] = kb;
       }
@@ -302,11 +302,15 @@ __device__ __forceinline__ double prodrates_wave(tchar* tab, TGR_LDS RhsLds<NEQ>
     if (rd == 0 && ((i1 >> 24) & 1)) {
       tchar* fo = tab + L.fo + lane * 8;
       const int fi = ld<int>(tab, L.fo + 12 * kRow + lane * 4);
-      double lnfc, k0f, k0b;
-      if (MODE == 2) {
-        lnfc = rc->lnfc[lane];
-        k0f = rc->k0f[lane];
-        k0b = rc->k0b[lane];
+      const double mv = S.M[(fi >> 8) & 255];
+      const double conc = mv > 0.0 ? mv : P / rt;
+      if constexpr (MODE == 2) {
+        const double lnfc = rc->lnfc[lane];
+        const double lfc = lnfc * kInvLn10;
+        const double ntmp = fma(-kTroeN1, lfc, kTroeN0);
+        const double cco = fma(-kTroeC1, lfc, -kTroeC0);
+        kf = troe(kf, rc->k0f[lane], conc, lnfc, ntmp, cco);
+        if ((i1 >> 25) & 1) kb = troe(kb, rc->k0b[lane], conc, lnfc, ntmp, cco);
       } else {
         const double invT = 1.0 / T;
         double fc = 0.0;
@@ -315,24 +319,23 @@ __device__ __forceinline__ double prodrates_wave(tchar* tab, TGR_LDS RhsLds<NEQ>
           const double x = -(ld<double>(fo, (9 + i) * kRow) * (((fi >> i) & 1) ? invT : T));
           fc = fma(ld<double>(fo, (6 + i) * kRow), fexp(x), fc);
         }
-        lnfc = flog(fc);
-        k0f = arrhenius(ld<double>(fo, 0), ld<double>(fo, kRow), ld<double>(fo, 2 * kRow), lt, irt);
-        k0b = ((i1 >> 25) & 1)
-                  ? arrhenius(ld<double>(fo, 3 * kRow), ld<double>(fo, 4 * kRow), ld<double>(fo, 5 * kRow), lt, irt)
-                  : 0.0;
-        if (MODE == 1) {
+        const double lnfc = flog(fc);
+        const double lfc = lnfc * kInvLn10;
+        const double ntmp = fma(-kTroeN1, lfc, kTroeN0);
+        const double cco = fma(-kTroeC1, lfc, -kTroeC0);
+        const double k0f = arrhenius(ld<double>(fo, 0), ld<double>(fo, kRow), ld<double>(fo, 2 * kRow), lt, irt);
+        if constexpr (MODE == 1) {
           rc->lnfc[lane] = lnfc;
           rc->k0f[lane] = k0f;
-          rc->k0b[lane] = k0b;
+        }
+        kf = troe(kf, k0f, conc, lnfc, ntmp, cco);
+        if ((i1 >> 25) & 1) {
+          const double k0b =
+              arrhenius(ld<double>(fo, 3 * kRow), ld<double>(fo, 4 * kRow), ld<double>(fo, 5 * kRow), lt, irt);
+          if constexpr (MODE == 1) rc->k0b[lane] = k0b;
+          kb = troe(kb, k0b, conc, lnfc, ntmp, cco);
         }
       }
-      const double mv = S.M[(fi >> 8) & 255];
-      const double conc = mv > 0.0 ? mv : P / rt;
-      const double lfc = lnfc * kInvLn10;
-      const double ntmp = fma(-kTroeN1, lfc, kTroeN0);
-      const double cco = fma(-kTroeC1, lfc, -kTroeC0);
-      kf = troe(kf, k0f, conc, lnfc, ntmp, cco);
-      if ((i1 >> 25) & 1) kb = troe(kb, k0b, conc, lnfc, ntmp, cco);
     }
     const double wf = kf * S.C[i0 & 255] * S.C[(i0 >> 8) & 255] * S.C[(i0 >> 16) & 255];
     const double wb = kb * S.C[i1 & 255] * S.C[(i1 >> 8) & 255] * S.C[(i1 >> 16) & 255];
