@@ -548,7 +548,7 @@ int run_surrogate(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, 
     hsub[n - 1] += dt - acc;  // exact sum
   }
   const int wpb = tgr_step_block_waves();
-  const int grid_s = persistent_grid(c, tgr_surr_occupancy(c->neq), p.ncell);
+  const int grid_s = persistent_grid(c, tgr_surr_occupancy(c->neq, c->surr_adaptive), p.ncell);
   const int grid_p = persistent_grid(c, tgr_step_occupancy(c->neq), p.ncell);
   // every buffer sized before the first launch (a resize frees memory)
   if ((rc = c->b_ws.ensure((size_t)std::max(grid_s, grid_p) * wpb * c->neq * 64 * sizeof(double))) ||

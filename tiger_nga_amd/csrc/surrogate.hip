@@ -86,7 +86,9 @@ __global__ void surr_reset_kernel(unsigned int* queue, unsigned int* rej_count, 
 
 // One cell: -1 wall cell (skipped), 0 accepted (fields written), 1..3
 // rejected (Newton, error estimate, enthalpy check; fields untouched).
-template <int NEQ>
+// AD: the error-controlled schedule (one kernel instance per schedule keeps
+// the other's code out of the register allocation)
+template <int NEQ, bool AD>
 __device__ __forceinline__ int surr_cell(const SurrParams& P, const DevMech& m, tchar* tab, TGR_LDS RhsLds<NEQ>* rl,
                                          TGR_LDS RateCache<NEQ>* rc, int64_t cell, double* ws,
                                          int& nit, int& nfe, int& nlu) {
@@ -114,7 +116,7 @@ __device__ __forceinline__ int surr_cell(const SurrParams& P, const DevMech& m, 
   Bdf<NEQ> b(m, tab, rl, ws, p.p_gp, p.rtol, p.atol);
   if (P.split_jac) b.rc = rc;
   int st;
-  if (P.adaptive) {
+  if constexpr (AD) {
     // the network's state at time t: linear in t between its stage times
     // (increment 0 at t = 0, the last output beyond the last time)
     const int nt = P.ntau;
@@ -232,7 +234,7 @@ __device__ __forceinline__ int surr_cell(const SurrParams& P, const DevMech& m, 
   return 0;
 }
 
-template <int NEQ>
+template <int NEQ, bool AD>
 __global__ __launch_bounds__(kSurrBlock) void surr_kernel(SurrParams P) {
   constexpr TabLayout L = tab_layout<NEQ>();
   __shared__ __attribute__((aligned(16))) char tab_s[L.bytes];
@@ -250,7 +252,7 @@ __global__ __launch_bounds__(kSurrBlock) void surr_kernel(SurrParams P) {
     const int64_t cell = (int64_t)(unsigned int)__builtin_amdgcn_readfirstlane((int)c);
     if (cell >= P.s.ncell) break;
     int nit = 0, nfe = 0, nlu = 0;
-    const int st = surr_cell<NEQ>(P, m, (tchar*)tab_s, (TGR_LDS RhsLds<NEQ>*)&rl[w],
+    const int st = surr_cell<NEQ, AD>(P, m, (tchar*)tab_s, (TGR_LDS RhsLds<NEQ>*)&rl[w],
                                   (TGR_LDS RateCache<NEQ>*)&rcs[w], cell, ws, nit, nfe, nlu);
     if (st < 0) continue;
     if (lane == 0) {
@@ -271,14 +273,20 @@ __global__ __launch_bounds__(kSurrBlock) void surr_kernel(SurrParams P) {
 
 using namespace tgr;
 
-extern "C" int tgr_surr_occupancy(int neq) {
+template <int NEQ>
+static hipError_t surr_occ(int* nb, bool adaptive) {
+  return adaptive ? hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, surr_kernel<NEQ, true>, kSurrBlock, 0)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, surr_kernel<NEQ, false>, kSurrBlock, 0);
+}
+
+extern "C" int tgr_surr_occupancy(int neq, int adaptive) {
   int nb = 0;
   hipError_t e = hipErrorInvalidValue;
   switch (neq) {
-    case 10: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, surr_kernel<10>, kSurrBlock, 0); break;
-    case 37: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, surr_kernel<37>, kSurrBlock, 0); break;
-    case 40: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, surr_kernel<40>, kSurrBlock, 0); break;
-    case 46: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, surr_kernel<46>, kSurrBlock, 0); break;
+    case 10: e = surr_occ<10>(&nb, adaptive != 0); break;
+    case 37: e = surr_occ<37>(&nb, adaptive != 0); break;
+    case 40: e = surr_occ<40>(&nb, adaptive != 0); break;
+    case 46: e = surr_occ<46>(&nb, adaptive != 0); break;
     default: break;
   }
   return e == hipSuccess ? nb : 0;
@@ -305,13 +313,22 @@ extern "C" int tgr_launch_surr_set_column(uint16_t* X, int64_t rows, int in_pad,
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
+template <int NEQ>
+static void launch_surr(const SurrParams& P, bool adaptive, int grid, hipStream_t s) {
+  if (adaptive)
+    surr_kernel<NEQ, true><<<dim3(grid), dim3(kSurrBlock), 0, s>>>(P);
+  else
+    surr_kernel<NEQ, false><<<dim3(grid), dim3(kSurrBlock), 0, s>>>(P);
+}
+
 extern "C" int tgr_launch_surr(const SurrParams* P, int neq, int grid, hipStream_t s) {
   hipLaunchKernelGGL(surr_reset_kernel, dim3(1), dim3(64), 0, s, P->s.queue, P->rej_count, P->stotals);
+  const bool ad = P->adaptive != 0;
   switch (neq) {
-    case 10: hipLaunchKernelGGL(surr_kernel<10>, dim3(grid), dim3(kSurrBlock), 0, s, *P); break;
-    case 37: hipLaunchKernelGGL(surr_kernel<37>, dim3(grid), dim3(kSurrBlock), 0, s, *P); break;
-    case 40: hipLaunchKernelGGL(surr_kernel<40>, dim3(grid), dim3(kSurrBlock), 0, s, *P); break;
-    case 46: hipLaunchKernelGGL(surr_kernel<46>, dim3(grid), dim3(kSurrBlock), 0, s, *P); break;
+    case 10: launch_surr<10>(*P, ad, grid, s); break;
+    case 37: launch_surr<37>(*P, ad, grid, s); break;
+    case 40: launch_surr<40>(*P, ad, grid, s); break;
+    case 46: launch_surr<46>(*P, ad, grid, s); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
