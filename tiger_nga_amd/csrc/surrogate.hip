@@ -86,9 +86,10 @@ __global__ void surr_reset_kernel(unsigned int* queue, unsigned int* rej_count, 
 
 // One cell: -1 wall cell (skipped), 0 accepted (fields written), 1..3
 // rejected (Newton, error estimate, enthalpy check; fields untouched).
-// AD: the error-controlled schedule (one kernel instance per schedule keeps
-// the other's code out of the register allocation)
-template <int NEQ, bool AD>
+// AD: the error-controlled schedule; NET: the stages start from the
+// network's guesses (one kernel instance per combination keeps the other
+// paths' code out of the register allocation)
+template <int NEQ, bool AD, bool NET>
 __device__ __forceinline__ int surr_cell(const SurrParams& P, const DevMech& m, tchar* tab, TGR_LDS RhsLds<NEQ>* rl,
                                          TGR_LDS RateCache<NEQ>* rc, int64_t cell, double* ws,
                                          int& nit, int& nfe, int& nlu) {
@@ -131,10 +132,10 @@ __device__ __forceinline__ int surr_cell(const SurrParams& P, const DevMech& m, 
       const double w = (k < nt && t1 > t0) ? (t - t0) / (t1 - t0) : 1.0;
       return sol0 + rho * osc * (v0 + w * (v1 - v0));
     };
-    st = b.sdirk_adaptive(sol, p.dt, P.hsub[0], P.h0_fac, gat, P.use_guess != 0, wacc, P.newton_max, P.newton_tol,
-                          P.max_att, P.grow, P.keep, nit);
+    st = b.sdirk_adaptive(sol, p.dt, P.hsub[0], P.h0_fac, gat, NET, wacc, P.newton_max, P.newton_tol, P.max_att,
+                          P.grow, P.keep, nit);
   } else {
-    st = b.sdirk_guided(sol, P.nsub, P.hsub, guess, P.use_guess != 0, wacc, P.newton_max, P.newton_tol, nit);
+    st = b.sdirk_guided(sol, P.nsub, P.hsub, guess, NET, wacc, P.newton_max, P.newton_tol, nit);
   }
   nfe = b.c_nfe + b.c_nfe_dq;
   nlu = b.c_nlu;
@@ -234,7 +235,7 @@ __device__ __forceinline__ int surr_cell(const SurrParams& P, const DevMech& m, 
   return 0;
 }
 
-template <int NEQ, bool AD>
+template <int NEQ, bool AD, bool NET>
 __global__ __launch_bounds__(kSurrBlock) void surr_kernel(SurrParams P) {
   constexpr TabLayout L = tab_layout<NEQ>();
   __shared__ __attribute__((aligned(16))) char tab_s[L.bytes];
@@ -252,7 +253,7 @@ __global__ __launch_bounds__(kSurrBlock) void surr_kernel(SurrParams P) {
     const int64_t cell = (int64_t)(unsigned int)__builtin_amdgcn_readfirstlane((int)c);
     if (cell >= P.s.ncell) break;
     int nit = 0, nfe = 0, nlu = 0;
-    const int st = surr_cell<NEQ, AD>(P, m, (tchar*)tab_s, (TGR_LDS RhsLds<NEQ>*)&rl[w],
+    const int st = surr_cell<NEQ, AD, NET>(P, m, (tchar*)tab_s, (TGR_LDS RhsLds<NEQ>*)&rl[w],
                                   (TGR_LDS RateCache<NEQ>*)&rcs[w], cell, ws, nit, nfe, nlu);
     if (st < 0) continue;
     if (lane == 0) {
@@ -275,8 +276,9 @@ using namespace tgr;
 
 template <int NEQ>
 static hipError_t surr_occ(int* nb, bool adaptive) {
-  return adaptive ? hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, surr_kernel<NEQ, true>, kSurrBlock, 0)
-                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, surr_kernel<NEQ, false>, kSurrBlock, 0);
+  // the instances differ in registers only a little; the default one decides
+  return adaptive ? hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, surr_kernel<NEQ, true, false>, kSurrBlock, 0)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, surr_kernel<NEQ, false, false>, kSurrBlock, 0);
 }
 
 extern "C" int tgr_surr_occupancy(int neq, int adaptive) {
@@ -315,10 +317,15 @@ extern "C" int tgr_launch_surr_set_column(uint16_t* X, int64_t rows, int in_pad,
 
 template <int NEQ>
 static void launch_surr(const SurrParams& P, bool adaptive, int grid, hipStream_t s) {
-  if (adaptive)
-    surr_kernel<NEQ, true><<<dim3(grid), dim3(kSurrBlock), 0, s>>>(P);
+  const bool net = P.use_guess != 0;
+  if (adaptive && net)
+    surr_kernel<NEQ, true, true><<<dim3(grid), dim3(kSurrBlock), 0, s>>>(P);
+  else if (adaptive)
+    surr_kernel<NEQ, true, false><<<dim3(grid), dim3(kSurrBlock), 0, s>>>(P);
+  else if (net)
+    surr_kernel<NEQ, false, true><<<dim3(grid), dim3(kSurrBlock), 0, s>>>(P);
   else
-    surr_kernel<NEQ, false><<<dim3(grid), dim3(kSurrBlock), 0, s>>>(P);
+    surr_kernel<NEQ, false, false><<<dim3(grid), dim3(kSurrBlock), 0, s>>>(P);
 }
 
 extern "C" int tgr_launch_surr(const SurrParams* P, int neq, int grid, hipStream_t s) {
