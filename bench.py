@@ -107,6 +107,11 @@ def traffic_record(workload, kernel="tgr::step_kernel"):
     return None
 
 
+def _progress(msg):
+    """A progress line on stderr (stdout carries only the JSON line)."""
+    print("[bench %s] %s" % (time.strftime("%H:%M:%S"), msg), file=sys.stderr, flush=True)
+
+
 def _time_steps(torch, a, fn, S_d, S0_d):
     """Seconds per step of fn over K steps (state restored before each)."""
     for _ in range(max(1, a.warmup)):
@@ -148,6 +153,7 @@ def bench_surrogate(a, torch, fc, g, R_d, S0_d, S_d, HR_d, CS_d, dt, P, mech, sp
     from tiger_nga_amd import flops
     guess_t = int(fc.options.get("surr_guess", 0))  # the tuned (or default) network use
     fc.set_option("surr_probe", 0)
+    _progress("guided step, forced on every call")
     t_forced = _time_steps(torch, a, guided, S_d, S0_d)
     S_d.copy_(S0_d)
     _, ss = fc.source_chemistry_surrogate(R_d, S_d, g, dt, P, P, HR_gp=HR_d, chemsrc_gp=CS_d, stream=sp)
@@ -156,18 +162,21 @@ def bench_surrogate(a, torch, fc, g, R_d, S0_d, S_d, HR_d, CS_d, dt, P, mech, sp
     # the other network setting on the same states: what the network's
     # guesses gain (or cost)
     fc.set_option("surr_guess", 1 - guess_t)
+    _progress("guided step with surr_guess %d" % (1 - guess_t))
     t_other = _time_steps(torch, a, guided, S_d, S0_d)
     S_d.copy_(S0_d)
     _, so = fc.source_chemistry_surrogate(R_d, S_d, g, dt, P, P, stream=sp)
     o_max = errors(Sp)[0]
     fc.set_option("surr_guess", guess_t)
     fc.set_option("surr_probe", 8)
+    _progress("guided step under the adaptive-use policy")
     t_adapt = _time_steps(torch, a, guided, S_d, S0_d)
     # relaxed states: both paths
     R2 = S0_d.clone()
     S_d.copy_(S0_d)
     fc.source_chemistry(R_d, S_d, g, dt, P, P, stream=sp)
     R2.copy_(S_d)
+    _progress("relaxed states: plain and guided steps")
     t_rp = _time_steps(torch, a, lambda: fc.source_chemistry(R_d, S_d, g, dt, P, P, HR_gp=HR_d, chemsrc_gp=CS_d,
                                                             stream=sp, sync=False), S_d, R2)
     Rp = S_d.clone()
@@ -311,6 +320,7 @@ def main():
         return bench_properties(a, torch, fc, g, S0_d, R_d, P, n, cells_label, mech, sp)
 
     stats = None
+    _progress("plain step, %s" % cells_label)
     for _ in range(max(1, a.warmup)):
         S_d.copy_(S0_d)
         stats = fc.source_chemistry(R_d, S_d, g, dt, P, P, HR_gp=HR_d, chemsrc_gp=CS_d, stream=sp, sync=True)
@@ -396,6 +406,7 @@ def main():
         out["roofline"] = gr
     if not a.no_cpu:
         nt = a.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        _progress("CPU oracle on %d cells, %d threads" % (a.cpu_cells, nt))
         out["cpu_baseline"], ref = cpu_baseline(a.workload, a.cpu_cells, nt)
         # both GPU paths against the FP64 oracle on the oracle's cells
         out["plain_vs_oracle"] = errors_vs_oracle(torch, S_plain, fc.nsp, ref)
