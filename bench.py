@@ -1,20 +1,28 @@
 #!/usr/bin/env python3
 """Chemistry-step benchmark on one MI355X.
 
-Workload (default ``gri30``): GRI-Mech 3.0 as shipped with the reference
-(grimech30F, 36 species, 219 reactions / 422 rate slots) on 64^3 = 262,144
-cells at 1 atm, dt = 5e-7 s, seeded synthetic states (tiger_nga_amd/inputs.py),
-plain FP64 path at the reference tolerances (rtol 1e-6, atol 1e-12,
-MAX_NSTEPS 10000).  One step = one finitechem_source_chemistry call over all
-cells with the fields resident in HBM (device entry point); before each step
-the fields are restored from a pristine device copy so that every step does
-the same work.
+Workload (default ``gri30_uneven``): GRI-Mech 3.0 as shipped with the
+reference (grimech30F, 36 species, 219 reactions / 422 rate slots) on 128^3 =
+2,097,152 cells at 1 atm, dt = 5e-7 s, seeded synthetic states
+(tiger_nga_amd/inputs.py) with uneven stiffness: 70 % near equilibrium
+(1800-2300 K), 20 % ignition front (1000-1300 K), 10 % cold mixing
+(300-600 K), randomly interleaved.  One step = one
+finitechem_source_chemistry call over all cells with the fields resident in
+HBM (device entry point); before each step the fields are restored from a
+pristine device copy so that every step does the same work.
 
-Prints one JSON line: cell-steps/s, the FP64 roofline of the step kernel
-(algorithmic flops from the integrator counters / kernel time, HIP events on
-the launch stream) and the CPU baseline: the C oracle (CVODE-semantics BDF and
-table-driven mechanism) on a bounded sample of the same cells, one cell loop
-per host thread.
+The metric compares the surrogate-guided step with the plain FP64 step at
+max species error <= 1e-6 against an FP64 reference: the GPU plain step at
+rtol 1e-10 / atol 1e-16 on the whole block (that solver matches the CPU
+oracle to rel 1e-8 in the parity tests), cross-checked by the CPU oracle at
+the same tolerances on a stratified sample (every stiffness class).  The
+guided path is the headline when it meets the bound; its max / RMS species
+and max T errors, overall and per class, are reported either way.
+
+Prints one JSON line: cell-steps/s, the FP64 roofline of the headline path's
+dominant kernel (algorithmic flops from the solver counters / kernel time,
+HIP events on the launch stream) and the CPU baseline: the C oracle on a
+bounded sample of the same cells, one serial cell loop per host thread.
 """
 import argparse
 import json
@@ -32,7 +40,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="gri30", choices=["h2", "gri30", "gri30_uneven", "dme39", "ethyred"])
+    ap.add_argument("--workload", default="gri30_uneven", choices=["h2", "gri30", "gri30_uneven", "dme39", "ethyred"])
+    ap.add_argument("--side-steps", type=int, default=2,
+                    help="timed steps of the side measurements (plain path when the guided one is the headline, "
+                         "forced guided solve, other network setting, relaxed states)")
+    ap.add_argument("--no-reference", action="store_true", help="skip the tight FP64 reference solve and errors")
+    ap.add_argument("--ref-sample", type=int, default=256,
+                    help="cells per stiffness class of the CPU-oracle cross-check of the tight reference")
     ap.add_argument("--cpu-cells", type=int, default=131072)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all usable host cores, at most 16")
     ap.add_argument("--no-cpu", action="store_true")
@@ -58,7 +72,31 @@ def fortran_empty(torch, shape, dev):
     return torch.empty_strided(shape, tuple(strides), dtype=torch.float64, device=dev)
 
 
-def cpu_baseline(workload, ncell, nthreads):
+def host_cores():
+    """Threads this process may use and the physical cores (core, package)
+    they sit on, from the kernel's topology files (as lscpu reports them)."""
+    cpus = sorted(os.sched_getaffinity(0))
+    phys = set()
+    for c in cpus:
+        base = "/sys/devices/system/cpu/cpu%d/topology/" % c
+        try:
+            with open(base + "core_id") as f1, open(base + "physical_package_id") as f2:
+                phys.add((f1.read().strip(), f2.read().strip()))
+        except OSError:
+            phys.add((str(c), "0"))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return len(cpus), len(phys), model
+
+
+def cpu_baseline(workload, ncell, nthreads, phys_cores, model):
     import numpy as np
 
     from oracle import pyoracle
@@ -69,29 +107,55 @@ def cpu_baseline(workload, ncell, nthreads):
     ncell = min(ncell, Y.shape[0])
     sol = inputs.packed_state(Y[:ncell], T[:ncell], rho[:ncell])
     t0 = time.perf_counter()
-    result = pyoracle.integrate_batch(mech, sol, dt, P, rtol=1e-6, atol=1e-12, nthreads=nthreads)
+    pyoracle.integrate_batch(mech, sol, dt, P, rtol=1e-6, atol=1e-12, nthreads=nthreads)
     el = time.perf_counter() - t0
     n1 = max(16, ncell // 64)
     t0 = time.perf_counter()
     pyoracle.integrate_batch(mech, sol[:n1], dt, P, rtol=1e-6, atol=1e-12, nthreads=1)
     el1 = time.perf_counter() - t0
-    rec = {"value": ncell / el, "unit": "cell-steps/s", "cores": nthreads, "kind": "port",
-           "sample": "first %d cells of the %s workload (same seeded states), C oracle BDF at rtol 1e-6, "
-                     "%d threads x serial cell loop; %.2f s wall" % (ncell, workload, nthreads, el),
-           "single_core_value": n1 / el1}
-    # oracle result as mass fractions and T, for the error of the GPU paths
-    rho = result[:, :-1].sum(axis=1)
-    return rec, np.concatenate([result[:, :-1] / rho[:, None], (result[:, -1] / rho)[:, None]], axis=1)
+    return {"value": ncell / el, "unit": "cell-steps/s", "cores": nthreads, "kind": "port",
+            "sample": "first %d cells of the %s workload (same seeded states), C oracle (CVODE-semantics BDF, "
+                      "table-driven mechanism) at rtol 1e-6 atol 1e-12, %d threads x serial cell loop; %.2f s wall"
+                      % (ncell, workload, nthreads, el),
+            "single_core_value": n1 / el1, "host_physical_cores": phys_cores, "host_cpu": model}
 
 
-def errors_vs_oracle(torch, S, nsp, ref):
-    """Max / RMS mass-fraction error and max T error of the first len(ref)
-    interior cells of a Fortran-strided (nx, ny, nz, nscalar) result."""
-    n = ref.shape[0]
-    got = torch.stack([S[..., k].permute(2, 1, 0).reshape(-1)[:n] for k in range(nsp + 1)], dim=1).cpu().numpy()
-    dY = abs(got[:, :nsp] - ref[:, :nsp])
-    return {"cells": n, "max_species_err": float(dY.max()), "rms_species_err": float((dY ** 2).mean() ** 0.5),
-            "max_T_err_K": float(abs(got[:, nsp] - ref[:, nsp]).max())}
+def stratified_cells(T0, per_class):
+    """Indices of up to per_class cells of each temperature class of the
+    workload (cold 300-600 K, ignition front 1000-1300 K, near equilibrium
+    >= 1800 K, the rest), first in cell order: a sample that holds every
+    stiffness class."""
+    import numpy as np
+    classes = {"cold": (T0 >= 300) & (T0 <= 600), "ignition": (T0 >= 1000) & (T0 <= 1300),
+               "equilibrium": T0 >= 1800}
+    other = ~(classes["cold"] | classes["ignition"] | classes["equilibrium"])
+    classes["other"] = other
+    return {k: np.nonzero(v)[0][:per_class] for k, v in classes.items() if v.any()}
+
+
+def interior_rows(torch, S, nvar):
+    """[cells, nvar] view of a Fortran-strided (nx, ny, nz, nscalar) field, cell order i fastest."""
+    return torch.stack([S[..., k].permute(2, 1, 0).reshape(-1) for k in range(nvar)], dim=1)
+
+
+def errors(torch, got, ref, nsp, classes=None):
+    """Max / RMS mass-fraction error and max T error of [cells, nsp + 1] rows
+    (device tensors or numpy arrays), overall and per class (index arrays)."""
+    import numpy as np
+    if not isinstance(got, np.ndarray):
+        got = got.cpu().numpy()
+    if not isinstance(ref, np.ndarray):
+        ref = ref.cpu().numpy()
+
+    def one(g, r):
+        dY = np.abs(g[:, :nsp] - r[:, :nsp])
+        return {"cells": int(g.shape[0]), "max_species_err": float(dY.max()),
+                "rms_species_err": float(np.sqrt((dY ** 2).mean())),
+                "max_T_err_K": float(np.abs(g[:, nsp] - r[:, nsp]).max())}
+    out = one(got, ref)
+    if classes:
+        out["by_class"] = {k: one(got[idx], ref[idx]) for k, idx in classes.items() if len(idx)}
+    return out
 
 
 def traffic_record(workload, kernel="tgr::step_kernel"):
@@ -112,102 +176,99 @@ def _progress(msg):
     print("[bench %s] %s" % (time.strftime("%H:%M:%S"), msg), file=sys.stderr, flush=True)
 
 
-def _time_steps(torch, a, fn, S_d, S0_d):
-    """Seconds per step of fn over K steps (state restored before each)."""
-    for _ in range(max(1, a.warmup)):
+def _time_steps(torch, fn, S_d, S0_d, steps, warmup):
+    """Seconds per step of fn over `steps` steps (state restored before each)."""
+    for _ in range(max(1, warmup)):
         S_d.copy_(S0_d)
         fn()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(a.steps):
+    for _ in range(steps):
         S_d.copy_(S0_d)
         fn()
     torch.cuda.synchronize()
-    return (time.perf_counter() - t0) / a.steps
+    return (time.perf_counter() - t0) / steps
 
 
-def bench_surrogate(a, torch, fc, g, R_d, S0_d, S_d, HR_d, CS_d, dt, P, mech, sp, stream, cells):
-    """The surrogate-guided step on the same block (S_d holds the plain FP64
-    result on entry): forced guided solve on every call, the adaptive policy
-    (guided solve only while it pays, with periodic probes), the same guided
-    solve without the network (Newton iterations compared), and both paths on
-    relaxed states (the block after one plain step, as in a running
-    simulation)."""
-    from tiger_nga_amd import surrogate
+ROOFLINE_UNIT = ("FP64 vector ALU: MI355X FP64 vector peak 78.6 TFLOP/s (the FP64 matrix peak is the same figure; "
+                 "these kernels issue no MFMA)")
+
+
+def bench_surrogate(a, torch, fc, g, R_d, S0_d, S_d, HR_d, CS_d, dt, P, mech, sp, cells):
+    """The surrogate-guided step on the same block: the headline timing (the
+    product call under its default adaptive-use policy, K steps), its state
+    and counters, and side measurements at a few steps each: the guided
+    solve forced on every call, the other network setting (what the network
+    gains), and both paths on relaxed states (the block after one plain step,
+    as in a running simulation)."""
+    from tiger_nga_amd import flops, surrogate
     path = surrogate.model_path(mech)
     if a.no_surrogate or not os.path.exists(path):
         return None
-    nsp = fc.nsp
-    Sp = S_d.clone()
+    side = max(1, min(a.steps, a.side_steps))
     sur = surrogate.Surrogate.load(path)
     sur.attach(fc)
 
     def guided():
         return fc.source_chemistry_surrogate(R_d, S_d, g, dt, P, P, HR_gp=HR_d, chemsrc_gp=CS_d, stream=sp, sync=False)
 
-    def errors(ref):
-        dY = (S_d[..., :nsp] - ref[..., :nsp]).abs()
-        dT = (S_d[..., nsp] - ref[..., nsp]).abs()
-        return float(dY.max()), float(dY.pow(2).mean().sqrt()), float(dT.max())
-
-    from tiger_nga_amd import flops
     guess_t = int(fc.options.get("surr_guess", 0))  # the tuned (or default) network use
-    fc.set_option("surr_probe", 0)
-    _progress("guided step, forced on every call")
-    t_forced = _time_steps(torch, a, guided, S_d, S0_d)
+    fc.set_option("surr_probe", 8)
+    _progress("guided step (default policy), %d + %d steps" % (a.warmup, a.steps))
+    t_head = _time_steps(torch, guided, S_d, S0_d, a.steps, a.warmup)
     S_d.copy_(S0_d)
     _, ss = fc.source_chemistry_surrogate(R_d, S_d, g, dt, P, P, HR_gp=HR_d, chemsrc_gp=CS_d, stream=sp)
-    e_max, e_rms, eT = errors(Sp)
-    state = S_d.clone()  # the guided result on the benchmark states, for the oracle comparison
+    state = S_d.clone()
+    fc.set_option("surr_probe", 0)
+    _progress("guided step forced on every call, %d steps" % side)
+    t_forced = _time_steps(torch, guided, S_d, S0_d, side, 1)
     # the other network setting on the same states: what the network's
     # guesses gain (or cost)
     fc.set_option("surr_guess", 1 - guess_t)
-    _progress("guided step with surr_guess %d" % (1 - guess_t))
-    t_other = _time_steps(torch, a, guided, S_d, S0_d)
+    _progress("guided step with surr_guess %d, %d steps" % (1 - guess_t, side))
+    t_other = _time_steps(torch, guided, S_d, S0_d, side, 1)
     S_d.copy_(S0_d)
     _, so = fc.source_chemistry_surrogate(R_d, S_d, g, dt, P, P, stream=sp)
-    o_max = errors(Sp)[0]
+    other_state = S_d.clone()
     fc.set_option("surr_guess", guess_t)
-    fc.set_option("surr_probe", 8)
-    _progress("guided step under the adaptive-use policy")
-    t_adapt = _time_steps(torch, a, guided, S_d, S0_d)
     # relaxed states: both paths
     R2 = S0_d.clone()
     S_d.copy_(S0_d)
     fc.source_chemistry(R_d, S_d, g, dt, P, P, stream=sp)
     R2.copy_(S_d)
-    _progress("relaxed states: plain and guided steps")
-    t_rp = _time_steps(torch, a, lambda: fc.source_chemistry(R_d, S_d, g, dt, P, P, HR_gp=HR_d, chemsrc_gp=CS_d,
-                                                            stream=sp, sync=False), S_d, R2)
+    _progress("relaxed states: plain and guided steps, %d each" % side)
+    t_rp = _time_steps(torch, lambda: fc.source_chemistry(R_d, S_d, g, dt, P, P, HR_gp=HR_d, chemsrc_gp=CS_d,
+                                                          stream=sp, sync=False), S_d, R2, side, 1)
     Rp = S_d.clone()
-    fc.set_option("surr_probe", 0)
-    t_rg = _time_steps(torch, a, guided, S_d, R2)
+    t_rg = _time_steps(torch, guided, S_d, R2, side, 1)
     S_d.copy_(R2)
     _, sr = fc.source_chemistry_surrogate(R_d, S_d, g, dt, P, P, HR_gp=HR_d, chemsrc_gp=CS_d, stream=sp)
-    r_max, r_rms, rT = errors(Rp)
+    nsp = fc.nsp
+    r_max = float((S_d[..., :nsp] - Rp[..., :nsp]).abs().max())
     fc.set_option("surr_probe", 8)
     fc.set_surrogate(None, None, None, None)
     gf = (flops.guided_flops(fc.info, ss, split=bool(fc.options.get("surr_split_jac", 1)))
           if ss["surrogate_ms"] > 0 else 0.0)
 
-    def net_case(t, st, emax):
+    def net_case(t, st):
         return {"value": cells / t, "ms_per_step": t * 1e3, "accepted_fraction": st["accepted"] / max(1, cells),
-                "newton_iters_per_cell": st["newton_iters"] / max(1, cells), "max_species_err": emax}
-    cur, oth = net_case(t_forced, ss, e_max), net_case(t_other, so, o_max)
+                "newton_iters_per_cell": st["newton_iters"] / max(1, cells)}
+    cur, oth = net_case(t_forced, ss), net_case(t_other, so)
     with_net, without_net = (cur, oth) if guess_t else (oth, cur)
     opts = {k: fc.options[k] for k in sorted(fc.options) if k.startswith("surr_")}
     return {
-        "_state": state,
-        "guided_roofline": {"bound": "mfma", "compute_unit": "FP64 vector ALU", "achieved": gf / (
-            ss["surrogate_ms"] * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-            "frac": gf / (ss["surrogate_ms"] * 1e-3) / PEAK_FP64, "flops": gf, "kernel_ms": ss["surrogate_ms"],
-            "kernel": "tgr::surr_kernel",
-            "note": "HIP events on the launch stream around the guided part (surr_kernel; with the network "
-                    "guess also the feature and network launches); flops of the FP64 guided solve"},
-        "value": cells / t_adapt, "ms_per_step": t_adapt * 1e3, "model": os.path.relpath(path, ROOT),
-        "policy": "adaptive: guided solve while its accepted fraction >= 0.15, else plain step with a guided "
+        "_state": state, "_other_state": other_state, "_guess": guess_t,
+        "guided_roofline": {"bound": "fp64-valu", "compute_unit": ROOFLINE_UNIT,
+                            "achieved": gf / (ss["surrogate_ms"] * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
+                            "unit": "TFLOP/s", "frac": gf / (ss["surrogate_ms"] * 1e-3) / PEAK_FP64, "flops": gf,
+                            "kernel_ms": ss["surrogate_ms"], "kernel": "tgr::surr_kernel",
+                            "note": "HIP events on the launch stream around the guided part (surr_kernel; with the "
+                                    "network guess also the feature and network launches); flops of the FP64 "
+                                    "guided solve (tiger_nga_amd/flops.py guided_flops)"},
+        "value": cells / t_head, "ms_per_step": t_head * 1e3, "model": os.path.relpath(path, ROOT),
+        "policy": "adaptive use: guided solve while its accepted fraction >= 0.15, else plain step with a guided "
                   "probe every 8 calls",
-        "forced_guided": {"value": cells / t_forced, "ms_per_step": t_forced * 1e3,
+        "forced_guided": {"value": cells / t_forced, "ms_per_step": t_forced * 1e3, "steps": side,
                           "accepted_fraction": ss["accepted"] / max(1, cells),
                           "rejected": {"newton": ss["rej_newton"], "error_estimate": ss["rej_error"],
                                        "enthalpy": ss["rej_enthalpy"]},
@@ -216,13 +277,13 @@ def bench_surrogate(a, torch, fc, g, R_d, S0_d, S_d, HR_d, CS_d, dt, P, mech, sp
                           "substeps_per_cell": ss["substeps"] / max(1, cells)},
         "options": opts,
         "network": {"used": bool(guess_t), "with_network_guess": with_net, "without_network_guess": without_net,
-                    "gain": without_net["ms_per_step"] / with_net["ms_per_step"]},
-        "max_species_err": e_max, "rms_species_err": e_rms, "max_T_err_K": eT,
-        "error_reference": "plain FP64 step of this run on the same states (rtol 1e-6, atol 1e-12)",
+                    "gain": without_net["ms_per_step"] / with_net["ms_per_step"],
+                    "newton_iteration_reduction": 1.0 - with_net["newton_iters_per_cell"] /
+                    max(1e-30, without_net["newton_iters_per_cell"])},
         "relaxed_states": {"what": "the same block after one plain step of dt (radical pools relaxed)",
-                           "plain_value": cells / t_rp, "guided_value": cells / t_rg, "speedup": t_rp / t_rg,
-                           "accepted_fraction": sr["accepted"] / max(1, cells), "max_species_err": r_max,
-                           "rms_species_err": r_rms, "max_T_err_K": rT},
+                           "steps": side, "plain_value": cells / t_rp, "guided_value": cells / t_rg,
+                           "speedup": t_rp / t_rg, "accepted_fraction": sr["accepted"] / max(1, cells),
+                           "max_species_err_vs_plain": r_max},
         "method": "per cell: 3-stage L-stable SDIRK3(2) with one frozen FP64 Jacobian, %s sub-steps, FP64 "
                   "Newton from %s, embedded error <= surr_tol_y in Y, element-conserving projection, "
                   "enthalpy-consistent T; rejected cells by the plain FP64 step"
@@ -254,7 +315,7 @@ def bench_properties(a, torch, fc, g, S_d, R_d, P, n, cells_label, mech, sp):
            "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": el / a.steps * 1e3,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": "%s density + Wilke viscosity + diffusivities, %s" % (mech, cells_label)},
-           "roofline": {"bound": "mfma", "compute_unit": "FP64 vector ALU", "achieved": F / ks / 1e12,
+           "roofline": {"bound": "fp64-valu", "compute_unit": ROOFLINE_UNIT, "achieved": F / ks / 1e12,
                         "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": F / ks / PEAK_FP64, "traffic": None,
                         "kernel_ms": ks * 1e3, "flops_per_step": F, "algorithmic_bytes_per_step": B,
                         "hbm_frac_at_algorithmic_bytes": B / ks / PEAK_HBM}}
@@ -277,6 +338,7 @@ def main():
 
     dev = "cuda:0"
     torch.cuda.set_device(0)
+    T0 = None
     if a.data_file:
         from tiger_nga_amd import ngadata
         from tiger_nga_amd.mech import meta
@@ -295,6 +357,7 @@ def main():
         workload = "data file %s" % os.path.basename(a.data_file)
         a.no_cpu = True
     else:
+        _progress("generating the %s states" % a.workload)
         mech, Y, T, rho, P, dt = inputs.states(a.workload, seed=0)
         n = Y.shape[0]
         nx = int(round(n ** (1.0 / 3.0)))
@@ -305,6 +368,7 @@ def main():
         S0_d = fortran_empty(torch, SC_.shape, dev)
         R_d.copy_(torch.from_numpy(RHO_))
         S0_d.copy_(torch.from_numpy(SC_))
+        T0 = T.copy()
         del RHO_, SC_, Y, T, rho
         cells_label = "%d^3 = %d cells" % (nx, n)
         workload = a.workload
@@ -318,18 +382,21 @@ def main():
     sp = stream.cuda_stream
     if a.kernel == "properties":
         return bench_properties(a, torch, fc, g, S0_d, R_d, P, n, cells_label, mech, sp)
-
+    nsp = fc.nsp
+    have_guided = not a.no_surrogate and os.path.exists(
+        __import__("tiger_nga_amd.surrogate", fromlist=["model_path"]).model_path(mech))
+    # the plain step: K steps when it is the headline, else a few (side measurement)
+    k_plain = max(1, min(a.steps, a.side_steps)) if have_guided else a.steps
     stats = None
-    _progress("plain step, %s" % cells_label)
+    _progress("plain step, %s, %d + %d steps" % (cells_label, a.warmup, k_plain))
     for _ in range(max(1, a.warmup)):
         S_d.copy_(S0_d)
         stats = fc.source_chemistry(R_d, S_d, g, dt, P, P, HR_gp=HR_d, chemsrc_gp=CS_d, stream=sp, sync=True)
     torch.cuda.synchronize()
-
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(k_plain)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for k in range(a.steps):
+    for k in range(k_plain):
         S_d.copy_(S0_d)
         evs[k][0].record(stream)
         fc.source_chemistry(R_d, S_d, g, dt, P, P, HR_gp=HR_d, chemsrc_gp=CS_d, stream=sp, sync=False)
@@ -338,80 +405,134 @@ def main():
     el = time.perf_counter() - t0
     kms = [e0.elapsed_time(e1) for e0, e1 in evs]
     kernel_s = sum(kms) / len(kms) / 1e3
-
     info = fc.info
     F = flops.step_flops(info, stats)
     achieved = F / kernel_s
     cells = stats["cells"]
+    plain_value = cells * k_plain / el
+    plain_roofline = {"bound": "fp64-valu", "compute_unit": ROOFLINE_UNIT, "achieved": achieved / 1e12,
+                      "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
+                      "kernel": "tgr::step_kernel", "kernel_ms": kernel_s * 1e3, "flops_per_step": F,
+                      "algorithmic_bytes_per_step": flops.step_bytes(info, cells)}
+    tr = None if a.data_file else traffic_record(workload)
+    if tr:
+        plain_roofline["traffic"] = tr["hbm_bytes_per_launch"]
+        plain_roofline["traffic_source"] = tr["source"]
+    S_plain = S_d.clone()
     metric = "GRI-3.0 chem cell-steps/s, 1 GPU: surrogate-guided vs plain FP64 at err<=1e-6"
     if mech != "gri30":
         metric = metric.replace("GRI-3.0", {"h2": "HYDROGENF", "dme39": "DME39F", "ethyred": "EthyRedF"}[mech])
     out = {
-        "metric": metric,
-        "value": cells * a.steps / el,
-        "unit": "cell-steps/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": el / a.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
+        "metric": metric, "value": plain_value, "unit": "cell-steps/s", "n_gpus": 1, "steps": k_plain,
+        "warmup": a.warmup, "ms_per_step": el / k_plain * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
         "data": ("fields of %s" % a.data_file) if a.data_file else
                 "synthetic seeded states (no case data ships with the reference)",
         "config": {"workload": "%s plain FP64 step, %s, dt %.1e s, P %.0f Pa, rtol 1e-6 atol 1e-12"
                                % (workload, cells_label, dt, P), "mechanism": mech, "path": "plain_fp64",
-                   "species": info["nsp"], "reactions": info["nrxn"], "rate_slots": info["nslot"],
+                   "cells": cells, "species": info["nsp"], "reactions": info["nrxn"], "rate_slots": info["nslot"],
                    "tuned": tuned or "defaults (no tuning record for this device)"},
-        "roofline": {"bound": "mfma", "compute_unit": "FP64 vector ALU (MI355X FP64 vector peak = FP64 matrix "
-                                                       "peak = 78.6 TFLOP/s)",
-                     "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64, "traffic": None,
-                     "kernel_ms": kernel_s * 1e3, "flops_per_step": F,
-                     "algorithmic_bytes_per_step": flops.step_bytes(info, cells)},
+        "roofline": plain_roofline,
         "solver": {k: stats[k] for k in ("nst", "nfe", "nfe_dq", "nje", "nsetups", "nni", "netf", "ncfn", "nlu",
                                          "failed")},
     }
-    S_plain = S_d.clone()
-    sur = bench_surrogate(a, torch, fc, g, R_d, S0_d, S_d, HR_d, CS_d, dt, P, mech, sp, stream, cells)
+    # the FP64 reference the errors are measured against
+    S_ref = None
+    if not a.no_reference:
+        _progress("tight FP64 reference: plain step at rtol 1e-10 atol 1e-16 on the whole block")
+        ft = FiniteChem(mech, 1e-10, 1e-16, 200000)
+        S_d.copy_(S0_d)
+        t0 = time.perf_counter()
+        st_ref = ft.source_chemistry(R_d, S_d, g, dt, P, P, stream=sp, sync=True)
+        ref_s = time.perf_counter() - t0
+        ft.close()
+        S_ref = S_d.clone()
+        out["reference"] = {"what": "GPU plain FP64 step at rtol 1e-10, atol 1e-16, max 200000 steps, whole block",
+                            "failed": st_ref["failed"], "steps_per_cell": st_ref["nst"] / max(1, cells),
+                            "seconds": ref_s}
+    classes = stratified_cells(T0, a.ref_sample) if T0 is not None else None
+    idx_all = np.concatenate(list(classes.values())) if classes else None
+    ref_rows = interior_rows(torch, S_ref, nsp + 1) if S_ref is not None else None
+    if ref_rows is not None:
+        full_cls = ({k: torch.from_numpy(np.nonzero(v)[0]).to(dev) for k, v in {
+            "cold": (T0 >= 300) & (T0 <= 600), "ignition": (T0 >= 1000) & (T0 <= 1300),
+            "equilibrium": T0 >= 1800}.items() if v.any()} if T0 is not None else None)
+
+        def vs_ref(S):
+            rows = interior_rows(torch, S, nsp + 1)
+            d = (rows[:, :nsp] - ref_rows[:, :nsp]).abs()
+            dT = (rows[:, nsp] - ref_rows[:, nsp]).abs()
+            res = {"cells": int(rows.shape[0]), "max_species_err": float(d.max()),
+                   "rms_species_err": float(d.pow(2).mean().sqrt()), "max_T_err_K": float(dT.max())}
+            if full_cls:
+                res["by_class"] = {k: {"cells": int(ix.numel()), "max_species_err": float(d[ix].max()),
+                                       "max_T_err_K": float(dT[ix].max())} for k, ix in full_cls.items()}
+            return res
+        out["plain_vs_reference"] = vs_ref(S_plain)
+    sur = bench_surrogate(a, torch, fc, g, R_d, S0_d, S_d, HR_d, CS_d, dt, P, mech, sp, cells) if have_guided \
+        else None
     sur_state = None
     if sur is not None:
         sur_state = sur.pop("_state")
+        other_state = sur.pop("_other_state")
+        guess_t = sur.pop("_guess")
+        if ref_rows is not None:
+            sur["vs_reference"] = vs_ref(sur_state)
+            # config 4: the network's guesses against the FP64 reference
+            key = "without_network_guess" if guess_t else "with_network_guess"
+            sur["network"][key]["vs_reference"] = vs_ref(other_state)
         out["surrogate"] = sur
-        # the metric compares the surrogate-guided step with the plain FP64 one
-        # at max species error <= 1e-6: the guided path is the headline when
-        # it meets that bound, else the plain path
-    tr = None if a.data_file else traffic_record(a.workload)
-    if tr:
-        out["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
-        out["roofline"]["traffic_source"] = tr["source"]
-    if sur is not None and sur["max_species_err"] <= 1e-6:
-        # the guided path is the headline: its throughput, and the roofline
-        # of its dominant kernel (the guided solve); the plain step's beside
-        out["plain_value"] = out["value"]
-        out["value"] = sur["value"]
-        out["ms_per_step"] = sur["ms_per_step"]
-        out["config"]["path"] = "surrogate_guided"
-        out["config"]["workload"] = ("%s chemistry step, SDIRK3(2)-guided with the plain FP64 step on rejected cells, "
-                                     "%s, dt %.1e s, P %.0f Pa, error <= 1e-6 against rtol 1e-6 atol 1e-12"
-                                     % (workload, cells_label, dt, P))
-        out["plain_roofline"] = out["roofline"]
-        gr = dict(sur["guided_roofline"])
-        gr["traffic"] = None
-        trs = None if a.data_file else traffic_record(a.workload, "tgr::surr_kernel")
+        rg = dict(sur["guided_roofline"])
+        rg["traffic"] = None
+        trs = None if a.data_file else traffic_record(workload, "tgr::surr_kernel")
         if trs:
-            gr["traffic"] = trs["hbm_bytes_per_launch"]
-            gr["traffic_source"] = trs["source"]
-        out["roofline"] = gr
+            rg["traffic"] = trs["hbm_bytes_per_launch"]
+            rg["traffic_source"] = trs["source"]
+        sur["guided_roofline"] = rg
+        # the guided path is the headline when it meets the metric's error bound
+        # against the FP64 reference (the plain step's max species error beside)
+        ok = ref_rows is not None and sur["vs_reference"]["max_species_err"] <= 1e-6
+        sur["meets_bound"] = ok
+        if ok:
+            out["plain_value"] = plain_value
+            out["plain_steps"] = k_plain
+            out["value"] = sur["value"]
+            out["steps"] = a.steps
+            out["ms_per_step"] = sur["ms_per_step"]
+            out["speedup_vs_plain"] = sur["value"] / plain_value
+            out["config"]["path"] = "surrogate_guided"
+            out["config"]["workload"] = (
+                "%s chemistry step, SDIRK3(2)-guided with the plain FP64 step on rejected cells, %s, dt %.1e s, "
+                "P %.0f Pa, max species error %.2e <= 1e-6 against the FP64 reference (rtol 1e-10 atol 1e-16)"
+                % (workload, cells_label, dt, P, sur["vs_reference"]["max_species_err"]))
+            out["plain_roofline"] = out["roofline"]
+            out["roofline"] = rg
     if not a.no_cpu:
-        nt = a.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        nt_aff, phys, model = host_cores()
+        nt = a.cpu_threads or min(16, nt_aff)
         _progress("CPU oracle on %d cells, %d threads" % (a.cpu_cells, nt))
-        out["cpu_baseline"], ref = cpu_baseline(a.workload, a.cpu_cells, nt)
-        # both GPU paths against the FP64 oracle on the oracle's cells
-        out["plain_vs_oracle"] = errors_vs_oracle(torch, S_plain, fc.nsp, ref)
-        if sur is not None:
-            out["surrogate"]["vs_oracle"] = errors_vs_oracle(torch, sur_state, fc.nsp, ref)
+        out["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_cells, nt, phys, model)
+        if classes is not None and ref_rows is not None:
+            # the tight CPU oracle on the stratified sample: checks the GPU
+            # reference, and gives every path's error against the oracle itself
+            from oracle import pyoracle
+            _progress("CPU oracle at rtol 1e-10 on a stratified sample of %d cells" % len(idx_all))
+            mech_, Y, T, rho, P_, dt_ = inputs.states(a.workload, seed=0)
+            sol = inputs.packed_state(Y[idx_all], T[idx_all], rho[idx_all])
+            res = pyoracle.integrate_batch(mech, sol, dt, P, rtol=1e-10, atol=1e-16, max_steps=200000, nthreads=nt)
+            r_rows = np.concatenate([res[:, :-1] / rho[idx_all, None], (res[:, -1] / rho[idx_all])[:, None]], axis=1)
+            cls_local, o = {}, 0
+            for k, v in classes.items():
+                cls_local[k] = np.arange(o, o + len(v))
+                o += len(v)
+            ix = torch.from_numpy(idx_all).to(dev)
+            pick = lambda S: interior_rows(torch, S, nsp + 1)[ix]
+            out["oracle_sample"] = {"what": "CPU oracle at rtol 1e-10 atol 1e-16 on up to %d cells per class"
+                                            % a.ref_sample,
+                                    "reference_vs_oracle": errors(torch, pick(S_ref), r_rows, nsp, cls_local),
+                                    "plain_vs_oracle": errors(torch, pick(S_plain), r_rows, nsp, cls_local)}
+            if sur_state is not None:
+                out["oracle_sample"]["guided_vs_oracle"] = errors(torch, pick(sur_state), r_rows, nsp, cls_local)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -8,6 +8,8 @@ outputs as fixtures (``<mech>_mech.npz``):
 * PRODRATES(CDOT, W, K, C, M, TEMP, PRESSURE)   (HYDROGENF.f90:4-277)
   -> per-slot rate coefficients K, rates W, third-body sums M, CDOT
 * COMPTHERMODATA(H, CP, T)                       (HYDROGENF.f90:514-673)
+  on a temperature ladder (H, CP) and at every state's T (H_state, CP_state:
+  with CDOT they fix finitechem_compute_rhs's temperature row)
 * GETMOLARMASS / GETSPECIESNAMES / GETNSPECIES / GETNREACTIONS
 * GETMUCOEFF / GETKOVEREPS                       (HYDROGENF.f90:453-496)
 
@@ -96,9 +98,16 @@ def main(argv=None):
             h, cp = np.zeros(nsp), np.zeros(nsp)
             lib.compthermodata_(_ptr(h), _ptr(cp), ctypes.byref(ctypes.c_double(tv)))
             H[s], CP[s] = h, cp
+        Hs = np.zeros((NSTATE, nsp))
+        CPs = np.zeros((NSTATE, nsp))
+        for s in range(NSTATE):
+            h, cp = np.zeros(nsp), np.zeros(nsp)
+            lib.compthermodata_(_ptr(h), _ptr(cp), ctypes.byref(ctypes.c_double(T[s])))
+            Hs[s], CPs[s] = h, cp
         out = os.path.join(HERE, "%s_mech.npz" % mech)
         np.savez_compressed(out, C=C, T=T, P=P, CDOT=CDOT, K=K, W=W, M=M, T_thermo=Tt, H=H, CP=CP,
-                            W_mol=W_mol, mucoeff=mucoeff, koveps=koveps, names=np.array(names), nslot=nslot)
+                            H_state=Hs, CP_state=CPs, W_mol=W_mol, mucoeff=mucoeff, koveps=koveps,
+                            names=np.array(names), nslot=nslot)
         print("%s: %d species, %d slots -> %s (%d bytes)" % (mech, nsp, nslot, out, os.path.getsize(out)))
 
 

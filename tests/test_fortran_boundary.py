@@ -49,6 +49,18 @@ def _need_driver():
         pytest.skip("Fortran caller not built (flang absent at build time)")
 
 
+def test_fortran_wrappers_check_optional_extents(tmp_path):
+    """A wrongly shaped mask, HR_gp or chemsrc_gp is refused by the Fortran
+    module with TGR_E_ARG (-1) before any data reaches the library."""
+    _need_driver()
+    _write_case(tmp_path / "in.bin", "h2", 3, 2, 2, 1, 1e-6, 1e-12)
+    r = subprocess.run([DRIVER, "h2", str(tmp_path / "in.bin"), str(tmp_path / "out.bin"), "--check-shapes"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    line = [ln for ln in r.stdout.splitlines() if ln.startswith("shape errors:")][0]
+    assert [int(v) for v in line.split(":")[1].split()] == [-1] * 6
+
+
 @pytest.mark.skipif(gpu_available(), reason="checks the no-device error path")
 def test_fortran_caller_fails_loudly_without_device(tmp_path):
     _need_driver()

@@ -1,0 +1,141 @@
+"""The benchmark workloads' own state distributions on the GPU against the
+tight CPU oracle, at 8^3 = 512 cells each (C ABI, host arrays):
+
+* ``gri30_uneven`` -- GRI-3.0 with 70 % near-equilibrium (1800-2300 K),
+  20 % ignition-front (1000-1300 K) and 10 % cold (300-600 K) cells, 1 atm,
+  dt 5e-7 s;
+* ``dme39`` -- DME39 at 10 atm, dt 2e-7 s, T 600-2200 K.
+
+Parity mode: GPU and oracle both at rtol 1e-10 / atol 1e-16 must agree to
+rel 1e-8 in Y where rtol sets the accuracy and 1e-6 K in T
+(finitechem_source_chemistry, combustion/finitechem.f90:962-1071).  Then the
+production paths -- the plain step at the reference tolerances (rtol 1e-6,
+atol 1e-12) and the surrogate-guided step with its tuned options -- are held
+to max |dY| <= 1e-6 against that tight oracle, with their max / RMS errors
+printed.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from tiger_nga_amd import inputs
+from tiger_nga_amd.finitechem import Grid
+
+pytestmark = pytest.mark.gpu
+
+NX = 8
+THREADS = max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    assert torch.cuda.is_available(), "GPU test without a HIP device"
+    return torch
+
+
+def _block(workload, seed=5):
+    mech, Y, T, rho, P, dt = inputs.states(workload, n=NX ** 3, seed=seed)
+    RHO_, SC_, isc_sc, isc_T = inputs.fortran_fields(Y, T, rho, NX, NX, NX, nover=1)
+    g = Grid.block(NX, NX, NX, SC_.shape[3], isc_sc, isc_T, nover=1)
+    return mech, Y, T, rho, P, dt, RHO_, SC_, g
+
+
+def _interior(SC_, nsp):
+    """[cells, nsp + 1] mass fractions and T of the interior, cell order i fastest."""
+    core = SC_[1:-1, 1:-1, 1:-1, :nsp + 1]
+    return np.stack([core[..., k].ravel(order="F") for k in range(nsp + 1)], axis=1)
+
+
+_TIGHT = {}
+
+
+def _tight_oracle(oracle, workload):
+    """Oracle at rtol 1e-10 / atol 1e-16 on the block's cells: [cells, nsp + 1]
+    (Y, T) and the packed initial state."""
+    if workload not in _TIGHT:
+        mech, Y, T, rho, P, dt, *_ = _block(workload)
+        sol = inputs.packed_state(Y, T, rho)
+        out, st = oracle.integrate_batch(mech, sol, dt, P, rtol=1e-10, atol=1e-16, max_steps=200000,
+                                         nthreads=THREADS, with_stats=True)
+        assert (st[:, -1] == 0).all()
+        _TIGHT[workload] = (np.concatenate([out[:, :-1] / rho[:, None], (out[:, -1] / rho)[:, None]], axis=1),
+                            sol, st)
+    return _TIGHT[workload]
+
+
+def _classes(T):
+    return {"cold (300-600 K)": int(((T >= 300) & (T <= 600)).sum()),
+            "ignition front (1000-1300 K)": int(((T >= 1000) & (T <= 1300)).sum()),
+            "near equilibrium (>= 1800 K)": int((T >= 1800).sum())}
+
+
+def _errors(got, ref, nsp):
+    dY = np.abs(got[:, :nsp] - ref[:, :nsp])
+    return float(dY.max()), float(np.sqrt((dY ** 2).mean())), float(np.abs(got[:, nsp] - ref[:, nsp]).max())
+
+
+@pytest.mark.parametrize("workload", ["gri30_uneven", "dme39"])
+def test_step_tight_vs_oracle(workload, oracle, torch_dev):
+    from tiger_nga_amd.finitechem import FiniteChem
+    mech, Y, T, rho, P, dt, RHO_, SC_, g = _block(workload)
+    cls = _classes(T)
+    if workload == "gri30_uneven":
+        assert min(cls.values()) >= 30, cls  # every stiffness class is in the sample
+    ref, sol0, ost = _tight_oracle(oracle, workload)
+    nsp = Y.shape[1]
+    fc = FiniteChem(mech, 1e-10, 1e-16, 200000)
+    S = SC_.copy(order="F")
+    st = fc.source_chemistry(RHO_, S, g, dt, P, P)
+    fc.close()
+    got = _interior(S, nsp)
+    assert st["failed"] == 0 and st["cells"] == NX ** 3
+    rel = np.abs(got[:, :nsp] - ref[:, :nsp]) / np.maximum(np.abs(ref[:, :nsp]), 1e-12)
+    dT = np.abs(got[:, nsp] - ref[:, nsp])
+    # where the tolerance is relative (rtol |rho Y| >= 10 atol) both solvers
+    # are accurate to rel ~1e-10; below it atol sets the accuracy
+    rel_dom = 1e-10 * rho[:, None] * np.abs(ref[:, :nsp]) >= 10.0 * 1e-16
+    ew = rho[:, None] * np.abs(got[:, :nsp] - ref[:, :nsp]) / (1e-10 * rho[:, None] * np.abs(ref[:, :nsp]) + 1e-16)
+    print("\n%s tight: cells %s; steps gpu %d oracle %d; max rel Y (rtol-governed) %.2e, max |dT| %.2e K, "
+          "max weighted %.2f" % (workload, cls, st["nst"], int(ost[:, 0].sum()), rel[rel_dom].max(), dT.max(),
+                                ew.max()))
+    for name, sel in (("cold", (T >= 300) & (T <= 600)), ("ignition", (T >= 1000) & (T <= 1300)),
+                      ("equilibrium", T >= 1800)):
+        if sel.any():
+            print("   %-11s cells %3d: max rel Y %.2e, max |dT| %.2e K" % (
+                name, sel.sum(), np.where(rel_dom[sel], rel[sel], 0.0).max(), dT[sel].max()))
+    assert rel[rel_dom].max() <= 1e-8
+    assert dT.max() <= 1e-6
+    assert ew.max() <= 100.0
+
+
+@pytest.mark.parametrize("workload", ["gri30_uneven", "dme39"])
+def test_production_paths_vs_tight_oracle(workload, oracle, torch_dev):
+    """The plain step at rtol 1e-6 / atol 1e-12 and the surrogate-guided step
+    (tuned options, trained network attached) against the tight oracle."""
+    from tiger_nga_amd import autotune, surrogate
+    from tiger_nga_amd.finitechem import FiniteChem
+    mech, Y, T, rho, P, dt, RHO_, SC_, g = _block(workload)
+    ref, _, _ = _tight_oracle(oracle, workload)
+    nsp = Y.shape[1]
+    fc = FiniteChem(mech, 1e-6, 1e-12)
+    tuned = autotune.apply(fc, inputs.WORKLOADS[workload][1])
+    S = SC_.copy(order="F")
+    st = fc.source_chemistry(RHO_, S, g, dt, P, P)
+    assert st["failed"] == 0
+    e_plain = _errors(_interior(S, nsp), ref, nsp)
+    print("\n%s plain rtol 1e-6: max |dY| %.2e, RMS %.2e, max |dT| %.2e K" % ((workload,) + e_plain))
+    assert e_plain[0] <= 1e-6
+    path = surrogate.model_path(mech)
+    assert os.path.exists(path), path
+    fc.load_surrogate(path)
+    fc.set_option("surr_probe", 0)
+    S = SC_.copy(order="F")
+    st, ss = fc.source_chemistry_surrogate(RHO_, S, g, dt, P, P)
+    fc.close()
+    assert st["failed"] == 0 and ss["accepted"] + ss["rejected"] == NX ** 3
+    e_g = _errors(_interior(S, nsp), ref, nsp)
+    print("%s guided (tuned %s): accepted %d / %d; max |dY| %.2e, RMS %.2e, max |dT| %.2e K"
+          % ((workload, tuned, ss["accepted"], NX ** 3) + e_g))
+    assert e_g[0] <= 1e-6

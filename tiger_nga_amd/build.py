@@ -51,12 +51,31 @@ def build_fortran(verbose=False):
     return DRIVER_OUT
 
 
-def build(verbose=False):
+OBJ_DIR = os.path.join(HERE, "_build")
+
+
+def build(verbose=False, jobs=None):
+    """Compiles each source to an object in parallel (the translation units
+    share no device code), then links the shared library."""
+    from concurrent.futures import ThreadPoolExecutor
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    base = command(extra=())[:-len(SOURCES) - 2]  # compiler and flags, without sources and output
+    base = [a for a in base if a not in ("-shared",)]
+    objs = [os.path.join(OBJ_DIR, s + ".o") for s in SOURCES]
+
+    def compile_one(k):
+        cmd = base + ["-c", os.path.join(CSRC, SOURCES[k]), "-o", objs[k]]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
+    n = jobs or min(len(SOURCES), max(1, min(8, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=n) as ex:
+        list(ex.map(compile_one, range(len(SOURCES))))
     tmp = OUT + ".tmp"
-    cmd = command(tmp)
+    link = [HIPCC, "--offload-arch=" + ARCH, "-fPIC", "-shared", *objs, "-o", tmp]
     if verbose:
-        print(" ".join(cmd))
-    subprocess.check_call(cmd)
+        print(" ".join(link))
+    subprocess.check_call(link)
     os.replace(tmp, OUT)
     build_fortran(verbose)
     return OUT

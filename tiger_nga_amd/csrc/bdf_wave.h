@@ -197,16 +197,25 @@ struct Bdf {
   // The Jacobian with the temperature-dependent factors evaluated once: with
   // T = y_T / rho, J(:, j) = df/dy_j|_T + df/dT|_y dT/dy_j, dT/dy_j = -T / rho
   // for a species and 1 / rho for y_T.  The species columns are difference
-  // quotients at fixed T (increments as in dq_jac) that reuse the base
-  // evaluation's rate constants and thermo (rc); df/dT is one difference
-  // quotient in T.  Stores J in jw and returns f(yv).
-  __device__ __forceinline__ double jac_split(double yv) {
+  // quotients at fixed T (increments as in dq_jac, from the current h) that
+  // reuse the base evaluation's rate constants and thermo (rc); df/dT is one
+  // difference quotient in T.  jac_split_base evaluates f(yv) and fills the
+  // rate cache; jac_split_cols (same yv, fy its result) stores J in jw.
+  __device__ __forceinline__ double jac_split_base(double yv) {
     constexpr int nsp = NEQ - 1;
     const double y0 = act ? yv : 0.0;
     const double rho = wave_sum(lane < nsp ? y0 : 0.0);
     const double T = bcast(y0, nsp) / rho;
     const double fy = rhs_call_T<NEQ, 1>(tab, S, rc, m.tmid, y0, p_gp, T);
     c_nfe++;
+    return fy;
+  }
+
+  __device__ __forceinline__ void jac_split_cols(double yv, double fy) {
+    constexpr int nsp = NEQ - 1;
+    const double y0 = act ? yv : 0.0;
+    const double rho = wave_sum(lane < nsp ? y0 : 0.0);
+    const double T = bcast(y0, nsp) / rho;
     const double srur = sqrt(UROUND);
     const double fnorm = wrms(fy);
     const double minInc = (fnorm != 0.0) ? (MIN_INC_MULT * fabs(h) * UROUND * n * fnorm) : 1.0;
@@ -227,6 +236,11 @@ struct Bdf {
     const double sy = -T / rho;
     for (int j = 0; j < nsp; ++j) fresh(jw)[j * kWave + lane] += sy * dfdT;
     fresh(jw)[nsp * kWave + lane] = dfdT / rho;
+  }
+
+  __device__ __forceinline__ double jac_split(double yv) {
+    const double fy = jac_split_base(yv);
+    jac_split_cols(yv, fy);
     return fy;
   }
 
@@ -784,10 +798,9 @@ struct Bdf {
     nst = 0;
     const double y0 = act ? yv : 0.0;
     set_ewt(y0);
-    h = h0;
     double f0;
     if (rc) {
-      f0 = jac_split(y0);
+      f0 = jac_split_base(y0);
     } else {
       f0 = f(y0);
       c_nfe++;
@@ -799,10 +812,11 @@ struct Bdf {
       h0 = (r0 > 0.0) ? h0_fac / r0 : dt;
       h0 = fmin(dt, fmax(dt * (1.0 / 1024.0), h0));
     }
-    if (!rc) {
-      h = h0;
-      dq_jac(y0, f0);
-    }
+    // the difference-quotient increments of either Jacobian follow the
+    // first step actually taken
+    h = h0;
+    if (rc) jac_split_cols(y0, f0);
+    else dq_jac(y0, f0);
     c_nje++;
     double yi = y0, t = 0.0, hh = fmin(h0, dt);
     double kl = act ? f0 : 0.0;  // derivative at yi (the last stage's, the method being stiffly accurate)

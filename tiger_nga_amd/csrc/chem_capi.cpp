@@ -224,9 +224,8 @@ int count_cells(const tgr_grid* g, const int* mask, int64_t mask_ld, int64_t nce
 }
 
 // persistent grid of a solver kernel: every wave of every resident workgroup pulls cells
-int persistent_grid(const tgr_chem* c, int occupancy, int64_t ncell) {
+int persistent_grid(const tgr_chem* c, int occupancy, int64_t ncell, int wpb) {
   const int per_cu = (c->grid_override <= 0 && occupancy > 0) ? occupancy : 1;
-  const int wpb = tgr_step_block_waves();
   const int64_t gs = c->grid_override > 0 ? c->grid_override : (int64_t)c->n_cu * per_cu;
   return (int)std::max<int64_t>(1, std::min<int64_t>(gs, (ncell + wpb - 1) / wpb));
 }
@@ -242,7 +241,7 @@ int run_step(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const
   StepParams p = step_params(c, dt, p_thermo, p_thermo_old, g, rho, sc, isc_sc0, isc_T0, mask, hr, chem);
   p.cell_list = list;
   p.nlist = nlist;
-  const int grid = persistent_grid(c, tgr_step_occupancy(c->neq), p.ncell);
+  const int grid = persistent_grid(c, tgr_step_occupancy(c->neq), p.ncell, tgr_step_block_waves());
   const int wpb = tgr_step_block_waves();
   int rc0 = c->b_ws.ensure((size_t)grid * wpb * c->neq * 64 * sizeof(double));
   if (rc0) return rc0;
@@ -547,11 +546,12 @@ int run_surrogate(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, 
     }
     hsub[n - 1] += dt - acc;  // exact sum
   }
-  const int wpb = tgr_step_block_waves();
-  const int grid_s = persistent_grid(c, tgr_surr_occupancy(c->neq, c->surr_adaptive), p.ncell);
-  const int grid_p = persistent_grid(c, tgr_step_occupancy(c->neq), p.ncell);
+  const int grid_s = persistent_grid(c, tgr_surr_occupancy(c->neq, c->surr_adaptive), p.ncell, tgr_surr_block_waves());
+  const int grid_p = persistent_grid(c, tgr_step_occupancy(c->neq), p.ncell, tgr_step_block_waves());
+  // the workspace serves both kernels: each wave of each block owns neq x 64 doubles
+  const size_t ws_waves = std::max((size_t)grid_s * tgr_surr_block_waves(), (size_t)grid_p * tgr_step_block_waves());
   // every buffer sized before the first launch (a resize frees memory)
-  if ((rc = c->b_ws.ensure((size_t)std::max(grid_s, grid_p) * wpb * c->neq * 64 * sizeof(double))) ||
+  if ((rc = c->b_ws.ensure(ws_waves * c->neq * 64 * sizeof(double))) ||
       (rc = c->b_x.ensure((size_t)rows * c->net_in_pad * 2)) ||
       (rc = c->b_guess.ensure((size_t)nst * rows * c->net_out_pad * 4)) ||
       (rc = c->b_rej.ensure((size_t)p.ncell * sizeof(int))))

@@ -18,6 +18,8 @@ int tgr_launch_temperature(const tgr::DevMech* m, int64_t n, const double* Y, co
 int tgr_launch_props(const tgr::PropsParams* p, int neq, hipStream_t s);
 int tgr_launch_rhs_lanes(int neq, int64_t n, double p_gp, const double* y, double* f, hipStream_t s);
 int tgr_surr_occupancy(int neq, int adaptive);
+// waves per workgroup of the guided-solve kernel (its workspace share per block)
+int tgr_surr_block_waves(void);
 int tgr_launch_surr_features(const tgr::StepParams* p, int neq, int64_t rows, int in_pad, const float* shift,
                              const float* scale, float log_dt, uint16_t* X, hipStream_t s);
 int tgr_launch_surr_set_column(uint16_t* X, int64_t rows, int in_pad, int col, float value, hipStream_t s);

@@ -198,22 +198,33 @@ __device__ __forceinline__ int surr_cell(const SurrParams& P, const DevMech& m, 
     if (wave_max(lane < nsp ? fabs(dl) * wacc : 0.0) > 1.0) return 2;
     if (lane < nsp) sol += dl;
   }
-  // enthalpy check: T1 from h(Y0, T0) = h(Y1, T1), Newton from the solved T
-  const double T0 = bcast(sol0, nsp) / rho;
+  // enthalpy check.  The right-hand side normalises by its own density
+  // rho_gp = sum(rho Y) (finitechem.f90:121-133), not by RHO_: the quantity
+  // its energy equation conserves is h(sol / rho_gp, sol_T / rho_gp), which
+  // differs from h(sol / RHO_, sol_T / RHO_) when sum(Y) /= 1.  T1' from
+  // h(Y0', T0') = h(Y1', T1') in those variables (Newton from the solved
+  // T'), then mapped back to the unpacked temperature sol_T / RHO_ by
+  // rho_gp / RHO_.
+  const double rg = wave_sum(lane < nsp ? sol0 : 0.0);
+  const double T0 = bcast(sol0, nsp) / rg;
   double hk, cpk;
   thermo_lane<NEQ>(tab, m.tmid, T0, lane, hk, cpk);
-  const double h0 = wave_sum(lane < nsp ? (sol0 / rho) * hk : 0.0);
-  const double y1 = lane < nsp ? sol / rho : 0.0;
-  const double Ts = bcast(sol, nsp) / rho;
-  double Th = Ts;
+  const double h0 = wave_sum(lane < nsp ? (sol0 / rg) * hk : 0.0);
+  const double y1g = lane < nsp ? sol / rg : 0.0;
+  const double Tsg = bcast(sol, nsp) / rg;
+  double Thg = Tsg;
   for (int it = 0; it < 8; ++it) {
-    thermo_lane<NEQ>(tab, m.tmid, Th, lane, hk, cpk);
-    const double hs = wave_sum(y1 * hk);
-    const double cs = wave_sum(y1 * cpk);
+    thermo_lane<NEQ>(tab, m.tmid, Thg, lane, hk, cpk);
+    const double hs = wave_sum(y1g * hk);
+    const double cs = wave_sum(y1g * cpk);
     const double dT = (hs - h0) / cs;
-    Th -= dT;
-    if (!(fabs(dT) > 1.0e-12 * Th)) break;
+    Thg -= dT;
+    if (!(fabs(dT) > 1.0e-12 * Thg)) break;
   }
+  const double to_unpacked = rg / rho;
+  const double Th = Thg * to_unpacked;
+  const double Ts = Tsg * to_unpacked;
+  const double y1 = lane < nsp ? sol / rho : 0.0;
   if (!(fabs(Th - Ts) <= P.tol_t)) return 3;
   // unpack and diagnostics as the plain path (finitechem.f90:1041-1058)
   const double soln = lane == nsp ? rho * Th : sol;
@@ -273,6 +284,8 @@ __global__ __launch_bounds__(kSurrBlock) void surr_kernel(SurrParams P) {
 }  // namespace tgr
 
 using namespace tgr;
+
+extern "C" int tgr_surr_block_waves(void) { return kSurrWaves; }
 
 template <int NEQ>
 static hipError_t surr_occ(int* nb, bool adaptive) {

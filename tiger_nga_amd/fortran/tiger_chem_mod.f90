@@ -205,8 +205,7 @@ contains
     if (present(HR_gp)) ph = c_loc(HR_gp)
     if (present(chemsrc_gp)) pc = c_loc(chemsrc_gp)
     if (present(stats)) ps = c_loc(stats)
-    if (size(RHO_, 1) /= g%imaxo_ - g%imino_ + 1 .or. size(RHO_, 2) /= g%jmaxo_ - g%jmino_ + 1 .or. &
-        size(RHO_, 3) /= g%kmaxo_ - g%kmino_ + 1 .or. size(SC_, 4) /= g%nscalar) then
+    if (.not. extents_ok(g, RHO_, SC_, mask, HR_gp, chemsrc_gp)) then
        ierr = TGR_E_ARG
        return
     end if
@@ -244,13 +243,37 @@ contains
     if (present(HR_gp)) ph = c_loc(HR_gp)
     if (present(chemsrc_gp)) pc = c_loc(chemsrc_gp)
     if (present(stats)) ps = c_loc(stats)
-    if (size(RHO_, 1) /= g%imaxo_ - g%imino_ + 1 .or. size(RHO_, 2) /= g%jmaxo_ - g%jmino_ + 1 .or. &
-        size(RHO_, 3) /= g%kmaxo_ - g%kmino_ + 1 .or. size(SC_, 4) /= g%nscalar) then
+    if (.not. extents_ok(g, RHO_, SC_, mask, HR_gp, chemsrc_gp)) then
        ierr = TGR_E_ARG
        return
     end if
     ierr = tgr_chem_source_chemistry_surrogate(tiger_chem_ctx, dt, P_thermo, P_thermo_old, g, c_loc(RHO_), &
          c_loc(SC_), pm, ph, pc, ps, c_null_ptr)
   end subroutine tiger_chem_source_chemistry_surrogate
+
+  ! the extents of every array the C side reads or writes: RHO_ and the
+  ! leading extents of SC_, HR_gp and chemsrc_gp are the ghosted block,
+  ! chemsrc_gp has 3 planes, mask is (imino:imaxo, jmino:jmaxo); a wrong
+  ! shape would otherwise be read or written past its end
+  logical function extents_ok(g, RHO_, SC_, mask, HR_gp, chemsrc_gp)
+    type(tgr_grid), intent(in) :: g
+    real(c_double), intent(in) :: RHO_(:,:,:)
+    real(c_double), intent(in) :: SC_(:,:,:,:)
+    integer(c_int), intent(in), optional :: mask(:,:)
+    real(c_double), intent(in), optional :: HR_gp(:,:,:)
+    real(c_double), intent(in), optional :: chemsrc_gp(:,:,:,:)
+    integer :: n1, n2, n3
+    n1 = g%imaxo_ - g%imino_ + 1
+    n2 = g%jmaxo_ - g%jmino_ + 1
+    n3 = g%kmaxo_ - g%kmino_ + 1
+    extents_ok = size(RHO_, 1) == n1 .and. size(RHO_, 2) == n2 .and. size(RHO_, 3) == n3 .and. &
+         size(SC_, 1) == n1 .and. size(SC_, 2) == n2 .and. size(SC_, 3) == n3 .and. size(SC_, 4) == g%nscalar
+    if (present(mask)) extents_ok = extents_ok .and. size(mask, 1) == g%imaxo - g%imino + 1 .and. &
+         size(mask, 2) == g%jmaxo - g%jmino + 1
+    if (present(HR_gp)) extents_ok = extents_ok .and. size(HR_gp, 1) == n1 .and. size(HR_gp, 2) == n2 .and. &
+         size(HR_gp, 3) == n3
+    if (present(chemsrc_gp)) extents_ok = extents_ok .and. size(chemsrc_gp, 1) == n1 .and. &
+         size(chemsrc_gp, 2) == n2 .and. size(chemsrc_gp, 3) == n3 .and. size(chemsrc_gp, 4) == 3
+  end function extents_ok
 
 end module tiger_chem
