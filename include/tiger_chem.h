@@ -234,6 +234,12 @@ int tgr_chem_set_option_real(tgr_chem *ctx, const char *key, double value);
  *                 from a device work counter
  *   "rhs_mapping" tgr_chem_rhs_device: 1 (default) one state per lane (generated
  *                 straight-line code of the mechanism), 0 one state per wavefront
+ *   "jacobian"    0 (default): the Jacobian by difference quotients, one
+ *                 right-hand side per column, as CVODE's FCVDENSE builds it for
+ *                 the reference (finitechem.f90:1028); 1: the analytic Jacobian
+ *                 of the same right-hand side (every PRODRATES / GETLINDRATECOEFF
+ *                 / COMPTHERMODATA term differentiated, from the mechanism
+ *                 tables), in the plain step and in the guided solve
  *   "max_order"   BDF maximum order 1..5 (CVODE default 5)
  *   "max_steps"   MAX_NSTEPS
  *   "surr_nsub"   sub-steps of the surrogate-guided schedule, 1..8 (default 8)

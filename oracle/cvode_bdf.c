@@ -71,7 +71,7 @@ enum { CF_NONE = 0, CF_BAD_J = 1, CF_OTHER = 2 };
 typedef struct {
   const tgr_mech_tables *t;
   double p_gp;
-  int n, qmax;
+  int n, qmax, jac_an;
   double rtol, atol;
   double *zn[QMAX_ABS + 1];
   double *ewt, *acor, *y, *tempv, *ftemp, *fdq;
@@ -181,7 +181,8 @@ static int lsetup(bdf *b, int convfail) {
     b->st->nje++;
     b->nstlj = b->nst;
     b->jcur = 1;
-    dq_jac(b, b->zn[0], b->ftemp);
+    if (b->jac_an) orc_jacobian_t(b->t, b->p_gp, b->zn[0], b->fdq, b->J); /* analytic (option) */
+    else dq_jac(b, b->zn[0], b->ftemp);
   }
   for (int j = 0; j < n; ++j)
     for (int i = 0; i < n; ++i) {
@@ -683,6 +684,7 @@ static int ws_init(bdf_ws *w, const tgr_mech_tables *t, const orc_opts *o, doubl
   b->p_gp = p_gp;
   b->n = n;
   b->qmax = (o->max_order >= 1 && o->max_order <= QMAX_ABS) ? o->max_order : QMAX_ABS;
+  b->jac_an = o->jac == 1;
   b->rtol = o->rtol;
   b->atol = o->atol;
   b->max_steps = o->max_steps > 0 ? o->max_steps : 500;

@@ -31,6 +31,9 @@
  *                   forms (pure species, ideal gas).
  *   orc_temperature enthalpy -> temperature Newton inversion (no reference
  *                   routine), pinned by round trips.
+ *   orc_jacobian    analytic Jacobian of finitechem_compute_rhs (no reference
+ *                   routine; CVODE differences the RHS, finitechem.f90:1028),
+ *                   pinned by central differences of orc_rhs.
  */
 #pragma once
 #include <stdint.h>
@@ -49,6 +52,8 @@ typedef struct {
   double rtol, atol;  /* scalar tolerances (finitechem.f90:266-267 defaults 1e-6, 1e-12) */
   int max_steps;      /* MAX_NSTEPS (finitechem.f90:1025: 10000) */
   int max_order;      /* BDF maximum order (CVODE default 5) */
+  int jac;            /* 0: difference-quotient Jacobian (CVODE FCVDENSE, the reference's);
+                         1: the analytic Jacobian (orc_jacobian) */
 } orc_opts;
 
 /* per-cell integrator counters (same meaning as CVODE's) */
@@ -80,6 +85,11 @@ int orc_properties(int mech, const double *Y, double Tcell, double P_thermo, con
 int orc_transport_fits(int mech, double *mucoeff, double *koveps);
 /* finitechem_compute_rhs: sol = [rho*Y (nsp), rho*T], p_gp = 0.5(P_th+P_th_old) */
 int orc_rhs(int mech, double p_gp, const double *sol, double *rhs);
+/* analytic derivatives (no reference routine: CVODE differences the RHS):
+ * dcdC[j*nsp + k] = d cdot_k / d C_j, dcdT[k] = d cdot_k / dT at fixed C;
+ * J[j*neq + i] = d rhs_i / d sol_j of finitechem_compute_rhs, f = rhs */
+int orc_dprodrates(int mech, const double *C, double T, double P, double *cdot, double *dcdC, double *dcdT);
+int orc_jacobian(int mech, double p_gp, const double *sol, double *f, double *J);
 /* one cell, t: 0 -> dt, sol in/out (neq = nsp+1) */
 int orc_integrate(int mech, const orc_opts *o, double dt, double p_gp, double *sol, orc_stats *st);
 /* batch of packed cells: sol[cell*neq + i]; threads >= 1 */

@@ -10,3 +10,6 @@ int orc_prodrates_t(const tgr_mech_tables *t, const double *C, double T, double 
 void orc_thermo_t(const tgr_mech_tables *t, double T, double *h, double *cp);
 int orc_temperature_t(const tgr_mech_tables *t, const double *Y, double hmix, double T0, double *T);
 void orc_rhs_t(const tgr_mech_tables *t, double p_gp, const double *sol, double *rhs);
+int orc_dprodrates_t(const tgr_mech_tables *t, const double *C, double T, double P, double *cdot, double *dcdC,
+                     double *dcdT);
+void orc_jacobian_t(const tgr_mech_tables *t, double p_gp, const double *sol, double *f, double *J);

@@ -18,7 +18,7 @@ MECH_ID = {"h2": 0, "gri30": 1, "dme39": 2, "ethyred": 3}
 
 class Opts(ctypes.Structure):
     _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("max_steps", ctypes.c_int),
-                ("max_order", ctypes.c_int)]
+                ("max_order", ctypes.c_int), ("jac", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -62,6 +62,8 @@ def lib():
         L.orc_properties.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_void_p, _dp, _dp,
                                      _dp, _dp]
         L.orc_transport_fits.argtypes = [ctypes.c_int, _dp, _dp]
+        L.orc_jacobian.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
+        L.orc_dprodrates.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -119,21 +121,41 @@ def rhs(mech, sol, p_gp):
     return out
 
 
-def integrate(mech, sol, dt, p_gp, rtol=1e-6, atol=1e-12, max_steps=10000, max_order=5):
+def jacobian(mech, sol, p_gp):
+    """(f, J) with J[i, j] = d f_i / d sol_j, the analytic Jacobian of the RHS."""
+    sol = np.ascontiguousarray(sol, dtype=np.float64)
+    n = sol.shape[0]
+    f = np.zeros(n)
+    Jc = np.zeros((n, n))  # column-major in C: Jc[j, i]
+    lib().orc_jacobian(MECH_ID[mech], p_gp, _p(sol), _p(f), _p(Jc))
+    return f, Jc.T.copy()
+
+
+def dprodrates(mech, C, T, P):
+    """(cdot, dcdC[k, j] = d cdot_k / d C_j, dcdT[k])."""
+    C = np.ascontiguousarray(C, dtype=np.float64)
+    n = C.shape[0]
+    cd, dT = np.zeros(n), np.zeros(n)
+    dC = np.zeros((n, n))
+    lib().orc_dprodrates(MECH_ID[mech], _p(C), float(T), float(P), _p(cd), _p(dC), _p(dT))
+    return cd, dC.T.copy(), dT
+
+
+def integrate(mech, sol, dt, p_gp, rtol=1e-6, atol=1e-12, max_steps=10000, max_order=5, jac=0):
     s = np.array(sol, dtype=np.float64)
     st = Stats()
-    o = Opts(rtol, atol, max_steps, max_order)
+    o = Opts(rtol, atol, max_steps, max_order, jac)
     lib().orc_integrate(MECH_ID[mech], ctypes.byref(o), dt, p_gp, _p(s), ctypes.byref(st))
     return s, {n: getattr(st, n) for n, _ in Stats._fields_}
 
 
 def integrate_batch(mech, sol, dt, p_gp, rtol=1e-6, atol=1e-12, max_steps=10000, nthreads=1, max_order=5,
-                    with_stats=False):
+                    with_stats=False, jac=0):
     """sol[n, neq] packed cells, returned integrated (copy)."""
     s = np.ascontiguousarray(np.array(sol, dtype=np.float64))
     n = s.shape[0]
     st = (Stats * n)() if with_stats else None
-    o = Opts(rtol, atol, max_steps, max_order)
+    o = Opts(rtol, atol, max_steps, max_order, jac)
     lib().orc_integrate_batch(MECH_ID[mech], ctypes.byref(o), dt, p_gp, n, _p(s),
                               ctypes.cast(st, ctypes.c_void_p) if st is not None else None, nthreads)
     if with_stats:
@@ -143,14 +165,14 @@ def integrate_batch(mech, sol, dt, p_gp, rtol=1e-6, atol=1e-12, max_steps=10000,
 
 
 def source_chemistry(mech, RHO_, SC_, g, dt, p_thermo, p_thermo_old=None, mask=None, HR_gp=None, chemsrc_gp=None,
-                     rtol=1e-6, atol=1e-12, max_steps=10000, nthreads=1):
+                     rtol=1e-6, atol=1e-12, max_steps=10000, nthreads=1, jac=0):
     """In place on Fortran-ordered numpy fields; g: tiger_nga_amd.finitechem.Grid-like."""
     if p_thermo_old is None:
         p_thermo_old = p_thermo
     cg = Grid(g.imin_, g.imax_, g.jmin_, g.jmax_, g.kmin_, g.kmax_, g.imino_, g.imaxo_, g.jmino_, g.jmaxo_,
               g.kmino_, g.kmaxo_, g.imino_m, g.imaxo_m, g.jmino_m, g.jmaxo_m, g.nscalar, g.isc_sc, g.isc_T)
     st = Stats()
-    o = Opts(rtol, atol, max_steps, 5)
+    o = Opts(rtol, atol, max_steps, 5, jac)
     ptr = lambda a: None if a is None else a.ctypes.data
     lib().orc_source_chemistry(MECH_ID[mech], ctypes.byref(o), dt, p_thermo, p_thermo_old, ctypes.byref(cg),
                                _p(RHO_), _p(SC_), ptr(mask), ptr(HR_gp), ptr(chemsrc_gp), ctypes.byref(st), nthreads)

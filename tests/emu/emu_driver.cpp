@@ -27,6 +27,7 @@ struct tgr_emu_jac_args {
   double p_gp, rtol, atol, h;
   const double* sol;
   double *jdq, *jsplit, *f;
+  double *jan, *fan;  // analytic Jacobian and its base evaluation (may be null)
 };
 
 namespace {
@@ -63,7 +64,10 @@ struct Run {
     Run* r = (Run*)(((uintptr_t)hi << 32) | (uintptr_t)lo);
     for (int64_t c = 0; c < r->p->ncell; ++c) {
       int cnt[TGR_NCOUNT] = {0};
-      const int done = tgr::process_cell<NEQ>(*r->p, r->p->m, r->p->m.tab, &r->rl, c, r->zn[l], r->ws.data(), cnt);
+      const int done = r->p->jac ? tgr::process_cell<NEQ, true>(*r->p, r->p->m, r->p->m.tab, &r->rl, c, r->zn[l],
+                                                                 r->ws.data(), cnt)
+                                 : tgr::process_cell<NEQ, false>(*r->p, r->p->m, r->p->m.tab, &r->rl, c, r->zn[l],
+                                                                  r->ws.data(), cnt);
       if (l == 0 && r->counters)
         for (int q = 0; q < TGR_NCOUNT; ++q) r->counters[c * TGR_NCOUNT + q] = done ? cnt[q] : 0;
     }
@@ -132,6 +136,14 @@ struct JacRun {
         for (int k = 0; k < NEQ; ++k) a.jsplit[(c * NEQ + l) * NEQ + k] = r->ws[k * tgr::kWave + l];
         a.f[c * NEQ + l] = fs - f0;  // the base evaluations agree
       }
+      if (a.jan) {
+        tgr::Bdf<NEQ, true> ba(r->m, r->m.tab, &r->rl, r->ws.data(), a.p_gp, a.rtol, a.atol);
+        const double fa = ba.jac_analytic(yv);
+        if (l < NEQ) {
+          for (int k = 0; k < NEQ; ++k) a.jan[(c * NEQ + l) * NEQ + k] = r->ws[k * tgr::kWave + l];
+          a.fan[c * NEQ + l] = fa;
+        }
+      }
     }
     r->finished++;
     if (r->finished == 64) {
@@ -169,13 +181,13 @@ void run_jac(const tgr_emu_jac_args& a, const tgr::DevMech& m) {
 
 }  // namespace
 
-// Both Jacobians (dq_jac, jac_split) on n packed states sol[n][neq]
+// The Jacobians (dq_jac, jac_split and, with jan, jac_analytic) on n packed states sol[n][neq]
 // ([rho Y_k, rho T]) with step size h for the increments: jdq, jsplit
 // [n][neq][neq] (row-major), f[n][neq] the difference of the two base
 // right-hand sides (zero when the fixed-temperature evaluation reproduces
 // the plain one).
 extern "C" int emu_jac(const char* mech, long n, double p_gp, double rtol, double atol, double h, const double* sol,
-                       double* jdq, double* jsplit, double* f) {
+                       double* jdq, double* jsplit, double* f, double* jan, double* fan) {
   const tgr_mech_tables* t = find_mech(mech);
   if (!t) return -1;
   std::vector<char> blob;
@@ -186,7 +198,7 @@ extern "C" int emu_jac(const char* mech, long n, double p_gp, double rtol, doubl
   m.tab = blob.data();
   m.nsp = t->nsp;
   m.tmid = t->tmid;
-  tgr_emu_jac_args a{n, p_gp, rtol, atol, h, sol, jdq, jsplit, f};
+  tgr_emu_jac_args a{n, p_gp, rtol, atol, h, sol, jdq, jsplit, f, jan, fan};
   switch (t->nsp + 1) {
     case 10: run_jac<10>(a, m); break;
     case 37: run_jac<37>(a, m); break;
@@ -201,7 +213,7 @@ extern "C" int emu_jac(const char* mech, long n, double p_gp, double rtol, doubl
 // (mass fractions then T, overwritten), hr[ncell], chemsrc[3 * ncell],
 // counters[ncell * 10] (nst nfe nfe_dq nje nsetups nni netf ncfn nlu status).
 extern "C" int emu_step(const char* mech, long ncell, double* rho, double* sc, double dt, double p_gp, double rtol,
-                        double atol, int max_steps, int* counters, double* hr, double* chemsrc) {
+                        double atol, int max_steps, int* counters, double* hr, double* chemsrc, int jac) {
   const tgr_mech_tables* t = find_mech(mech);
   if (!t) return -1;
   std::vector<char> blob;
@@ -218,6 +230,7 @@ extern "C" int emu_step(const char* mech, long ncell, double* rho, double* sc, d
   p.atol = atol;
   p.max_steps = max_steps;
   p.qmax = 5;
+  p.jac = jac;
   p.nx = (int)ncell;
   p.ny = p.nz = 1;
   p.sx = 1;

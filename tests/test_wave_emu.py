@@ -22,10 +22,10 @@ def emu():
     lib = ctypes.CDLL(mod.build())
     vp = ctypes.c_void_p
     lib.emu_step.argtypes = [ctypes.c_char_p, ctypes.c_long, vp, vp, ctypes.c_double, ctypes.c_double,
-                             ctypes.c_double, ctypes.c_double, ctypes.c_int, vp, vp, vp]
+                             ctypes.c_double, ctypes.c_double, ctypes.c_int, vp, vp, vp, ctypes.c_int]
     lib.emu_lane_rhs.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_double, vp, vp]
     lib.emu_jac.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_double, ctypes.c_double, ctypes.c_double,
-                            ctypes.c_double, vp, vp, vp, vp]
+                            ctypes.c_double, vp, vp, vp, vp, vp, vp]
     return lib
 
 
@@ -38,7 +38,7 @@ def test_emulated_kernel_matches_oracle(emu, oracle, workload, n, rtol, atol):
     cnt = np.zeros((n, 10), dtype=np.int32)
     hr, cs = np.zeros(n), np.zeros(3 * n)
     rc = emu.emu_step(mech.encode(), n, rho.ctypes.data, sc.ctypes.data, dt, P, rtol, atol, 10000,
-                      cnt.ctypes.data, hr.ctypes.data, cs.ctypes.data)
+                      cnt.ctypes.data, hr.ctypes.data, cs.ctypes.data, 0)
     assert rc == 0
     ref, st = oracle.integrate_batch(mech, inputs.packed_state(Y, T, rho), dt, P, rtol=rtol, atol=atol,
                                      with_stats=True)
@@ -86,7 +86,7 @@ def _emu_run(emu, mech, Y, T, rho, P, dt, rtol, atol, max_steps):
     cnt = np.zeros((n, 10), dtype=np.int32)
     hr, cs = np.zeros(n), np.zeros(3 * n)
     rc = emu.emu_step(mech.encode(), n, rho.ctypes.data, sc.ctypes.data, dt, P, rtol, atol, max_steps,
-                      cnt.ctypes.data, hr.ctypes.data, cs.ctypes.data)
+                      cnt.ctypes.data, hr.ctypes.data, cs.ctypes.data, 0)
     assert rc == 0
     return sc, cnt
 
@@ -138,7 +138,7 @@ def test_split_jacobian_against_central_differences(emu, workload):
     js = np.zeros_like(jd)
     df = np.zeros((n, neq))
     assert emu.emu_jac(workload.encode(), n, P, 1e-6, 1e-12, dt / 255, sol.ctypes.data, jd.ctypes.data,
-                       js.ctypes.data, df.ctypes.data) == 0
+                       js.ctypes.data, df.ctypes.data, None, None) == 0
     assert np.all(df == 0.0)
 
     def rhs(s):
@@ -164,3 +164,54 @@ def test_split_jacobian_against_central_differences(emu, workload):
     # increment: rho T vs T)
     tc = np.abs(js[:, :, -1] - jd[:, :, -1]).max() / np.abs(jd[:, :, -1]).max()
     assert tc <= 1e-5, tc
+
+
+@pytest.mark.parametrize("workload", ["h2", "gri30_uneven", "dme39", "ethyred"])
+def test_analytic_jacobian_matches_oracle(emu, oracle, workload):
+    """Bdf::jac_analytic (the device code, run on the emulator) against the
+    oracle's analytic Jacobian, itself pinned by central differences
+    (tests/test_oracle_jacobian.py): entries within 1e-10 of their column's
+    scale plus 1e-12 of their row's, and the base evaluation equal to the
+    oracle's right-hand side to 1e-12 of its scale."""
+    mech, Y, T, rho, P, dt = inputs.states(workload, n=4, seed=3)
+    sol = np.ascontiguousarray(inputs.packed_state(Y, T, rho))
+    n, neq = sol.shape
+    jd = np.zeros((n, neq, neq))
+    js, ja = np.zeros_like(jd), np.zeros_like(jd)
+    df, fa = np.zeros((n, neq)), np.zeros((n, neq))
+    assert emu.emu_jac(mech.encode(), n, P, 1e-6, 1e-12, dt / 255, sol.ctypes.data, jd.ctypes.data,
+                       js.ctypes.data, df.ctypes.data, ja.ctypes.data, fa.ctypes.data) == 0
+    for c in range(n):
+        f, J = oracle.jacobian(mech, sol[c], P)
+        assert np.abs(fa[c] - f).max() <= 1e-12 * np.abs(f).max()
+        col = np.abs(J).max(axis=0, keepdims=True)
+        row = np.abs(J).max(axis=1, keepdims=True)
+        err = np.abs(ja[c] - J)
+        assert (err <= 1e-10 * col + 1e-12 * row).all(), (c, (err / (col + 1e-300)).max())
+
+
+@pytest.mark.parametrize("workload,n", [("h2", 4), ("gri30_uneven", 2), ("dme39", 1)])
+def test_emulated_kernel_analytic_jacobian_matches_oracle(emu, oracle, workload, n):
+    """The step with the analytic Jacobian (StepParams.jac = 1) against the
+    oracle's integrator with its analytic Jacobian: the same step sequence
+    (counters) on all but rounding-flipped cells, no difference-quotient
+    evaluations, and the results of cells on the same sequence equal to
+    rounding (tight tolerance)."""
+    mech, Y, T, rho, P, dt = inputs.states(workload, n=n, seed=5)
+    rtol, atol = 1e-10, 1e-16
+    sc = np.ascontiguousarray(np.concatenate([Y.T, T[None, :]], axis=0))
+    cnt = np.zeros((n, 10), dtype=np.int32)
+    hr, cs = np.zeros(n), np.zeros(3 * n)
+    assert emu.emu_step(mech.encode(), n, rho.ctypes.data, sc.ctypes.data, dt, P, rtol, atol, 100000,
+                        cnt.ctypes.data, hr.ctypes.data, cs.ctypes.data, 1) == 0
+    ref, st = oracle.integrate_batch(mech, inputs.packed_state(Y, T, rho), dt, P, rtol=rtol, atol=atol,
+                                     max_steps=100000, with_stats=True, jac=1)
+    assert (cnt[:, 9] == 0).all() and (st[:, 9] == 0).all()
+    assert (cnt[:, 2] == 0).all() and (cnt[:, 3] > 0).all()
+    same = (cnt[:, :9] == st[:, :9]).all(axis=1)
+    assert same.sum() >= n - 1, (cnt, st)
+    Yg, Yo = sc[:-1].T, ref[:, :-1] / rho[:, None]
+    rel = np.abs(Yg - Yo) / np.maximum(np.abs(Yo), 1e-12)
+    dom = rtol * rho[:, None] * np.abs(Yo) >= 10 * atol
+    assert rel[same][dom[same]].max(initial=0.0) <= 1e-8
+    assert np.abs(sc[-1] - ref[:, -1] / rho).max() < 1e-6

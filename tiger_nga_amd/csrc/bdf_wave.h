@@ -55,7 +55,76 @@ __device__ __forceinline__ void put(T (&a)[N], int i, T v) {
   for (int k = 0; k < N; ++k) a[k] = (i == k) ? v : a[k];
 }
 
+// Analytic Jacobian of finitechem_compute_rhs at y0 (this lane's component)
+// into jw (jw[c * 64 + lane] = J(lane, c)); returns f(y0).
+// With C_k = y_k / W_k, T = y_T / rho, rho = sum_k y_k (species) and
+// f_k = W_k CDOT_k(C, T), f_T = -g / cp_mix, g = sum_k h_k f_k:
+//   J(k, j)  = W_k dCDOT_k/dC_j / W_j + fT_k dT/dy_j,  fT_k = W_k dCDOT_k/dT
+//   J(T, j)  = -(dT/dy_j sum_k cp_k f_k + sum_k h_k J(k, j)) / cp_mix
+//              + g / cp_mix^2 dcp_mix/dy_j
+//   dT/dy_j  = -T / rho (species), 1 / rho (y_T);
+//   dcp_mix/dy_j = (cp_j - cp_mix) / rho (species) + dT/dy_j sum_k Y_k dcp_k/dT.
 template <int NEQ>
+__device__ __forceinline__ double jac_wave(tchar* tab, TGR_LDS RhsLds<NEQ>* S, double* jw, double tmid, double y0,
+                                           double p_gp) {
+  constexpr int nsp = NEQ - 1;
+  constexpr TabLayout L = tab_layout<NEQ>();
+  const int lane = lane_id();
+  const bool sp = lane < nsp;
+  const double rho = wave_sum(sp ? y0 : 0.0);
+  const double T = bcast(y0, nsp) / rho;
+  double hk, cpk, dcpk;
+  thermo_lane_d<NEQ>(tab, tmid, T, lane, hk, cpk, dcpk);
+  const double Wk = sp ? ld<double>(tab, L.w + lane * 8) : 0.0;
+  const double Ck = sp ? y0 * ld<double>(tab, L.iw + lane * 8) : 0.0;
+  const double cpm = wave_sum(sp ? (y0 / rho) * cpk : 0.0);
+  const double cpmT = wave_sum(sp ? (y0 / rho) * dcpk : 0.0);
+  const double ctot = wave_sum(Ck);
+  JacLane jl[L.rounds];
+  double dcdT;
+  const double cdot = prodrates_jac_base<NEQ>(tab, *S, Ck, ctot, T, p_gp, jl, dcdT);
+  const double fk = sp ? cdot * Wk : 0.0;
+  const double fTk = sp ? dcdT * Wk : 0.0;
+  const double tmp = wave_sum(-hk * fk);
+  const double A = wave_sum(cpk * fk);
+  const double fy = sp ? fk : (lane == nsp ? tmp / cpm : 0.0);
+  const double g = -tmp;
+  const double irho = 1.0 / rho;
+  const double dTs = -T * irho;
+  const double icpm = 1.0 / cpm;
+  const double gc2 = g * icpm * icpm;
+  TGR_GLOBAL double* jg = (TGR_GLOBAL double*)jw;  // stores only (fresh() needs a scalar pointer)
+  for (int j = 0; j < nsp; ++j) {
+    const double dc = jac_column<NEQ>(tab, *S, jl, j);
+    const double iWj = ld<double>(tab, L.iw + j * 8);
+    const double Jk = sp ? fma(Wk * dc, iWj, fTk * dTs) : 0.0;
+    const double hJ = wave_sum(hk * Jk);
+    const double cpj = bcast(cpk, j);
+    const double dcpm = fma(cpmT, dTs, (cpj - cpm) * irho);
+    const double JT = fma(gc2, dcpm, -fma(dTs, A, hJ) * icpm);
+    jg[j * kWave + lane] = sp ? Jk : (lane == nsp ? JT : 0.0);
+  }
+  {
+    const double Jk = sp ? fTk * irho : 0.0;
+    const double hJ = wave_sum(hk * Jk);
+    const double JT = fma(gc2, cpmT * irho, -fma(irho, A, hJ) * icpm);
+    jg[nsp * kWave + lane] = sp ? Jk : (lane == nsp ? JT : 0.0);
+  }
+  return fy;
+}
+
+// Out of line, like the right-hand side: its per-reaction registers do not
+// join the integrator's register allocation.
+template <int NEQ>
+__device__ __attribute__((noinline)) double jac_call(tchar* tab, TGR_LDS RhsLds<NEQ>* S, double* jw, double tmid,
+                                                     double y0, double p_gp) {
+  return jac_wave<NEQ>(tab, S, jw, tmid, y0, p_gp);
+}
+
+// AJ: the analytic Jacobian (jac_analytic) instead of difference quotients
+// (a compile-time choice: the difference-quotient instance carries no call to
+// it, and its register allocation is that of the plain solver)
+template <int NEQ, bool AJ = false>
 struct Bdf {
   const DevMech m;  // by value: fields stay in scalar registers
   tchar* tab;        // mechanism tables in LDS
@@ -244,6 +313,11 @@ struct Bdf {
     return fy;
   }
 
+  // Analytic Jacobian at yv into jw (jac_wave below); returns f(yv).
+  __device__ __forceinline__ double jac_analytic(double yv) {
+    return jac_call<NEQ>(tab, S, jw, m.tmid, act ? yv : 0.0, p_gp);
+  }
+
   __device__ __forceinline__ int lsetup(int convfail) {
     const double dgamma = fabs(gamma / gammap - 1.0);
     const bool jbad = (nst == 0) || (nst > nstlj + MSBJ) || ((convfail == CF_BAD_J) && (dgamma < DGMAX_J)) ||
@@ -254,7 +328,8 @@ struct Bdf {
       c_nje++;
       nstlj = nst;
       jcur = 1;
-      dq_jac(zn[0][lane], ftemp);
+      if constexpr (AJ) jac_analytic(zn[0][lane]);
+      else dq_jac(zn[0][lane], ftemp);
     }
     const TGR_GLOBAL double* jr = fresh(jw) + lane;
 #pragma unroll
@@ -799,7 +874,9 @@ struct Bdf {
     const double y0 = act ? yv : 0.0;
     set_ewt(y0);
     double f0;
-    if (rc) {
+    if constexpr (AJ) {
+      f0 = jac_analytic(y0);  // the Jacobian does not depend on the step size
+    } else if (rc) {
       f0 = jac_split_base(y0);
     } else {
       f0 = f(y0);
@@ -815,8 +892,10 @@ struct Bdf {
     // the difference-quotient increments of either Jacobian follow the
     // first step actually taken
     h = h0;
-    if (rc) jac_split_cols(y0, f0);
-    else dq_jac(y0, f0);
+    if constexpr (!AJ) {
+      if (rc) jac_split_cols(y0, f0);
+      else dq_jac(y0, f0);
+    }
     c_nje++;
     double yi = y0, t = 0.0, hh = fmin(h0, dt);
     double kl = act ? f0 : 0.0;  // derivative at yi (the last stage's, the method being stiffly accurate)
@@ -922,7 +1001,9 @@ struct Bdf {
     h = hs[0];
     const double y0 = act ? yv : 0.0;
     set_ewt(y0);
-    if (rc) {
+    if constexpr (AJ) {
+      jac_analytic(y0);
+    } else if (rc) {
       jac_split(y0);
     } else {
       const double f0 = f(y0);
@@ -984,7 +1065,7 @@ struct Bdf {
 // sol = [max(RHO*Y, 0), RHO*T], integrate 0 -> dt, unpack by dividing by
 // RHO_, heat release and H2O/CO/CO2 sources.  Returns 0 for a wall cell
 // (mask == 1, skipped), else 1 with the integrator counters in cnt.
-template <int NEQ>
+template <int NEQ, bool AJ = false>
 __device__ __forceinline__ int process_cell(const StepParams& p, const DevMech& m, tchar* tab, TGR_LDS RhsLds<NEQ>* rl,
                                             int64_t cell, TGR_LDS double (&zn_lds)[kQmaxAbs + 2][kWave], double* ws,
                                             int* cnt) {
@@ -1003,7 +1084,7 @@ __device__ __forceinline__ int process_cell(const StepParams& p, const DevMech& 
     if (lane < nsp) sol = fmax(rho * p.sc[(int64_t)(p.isc_sc0 + lane) * p.plane + off], 0.0);
     else if (lane == nsp) sol = rho * p.sc[(int64_t)p.isc_T0 * p.plane + off];
     const double sol0 = sol;
-    Bdf<NEQ> b(m, tab, rl, zn_lds, ws, p.p_gp, p.rtol, p.atol, p.qmax, p.max_steps);
+    Bdf<NEQ, AJ> b(m, tab, rl, zn_lds, ws, p.p_gp, p.rtol, p.atol, p.qmax, p.max_steps);
     const int st = b.integrate(p.dt, sol);
     // unpack (finitechem.f90:1041-1042) and diagnostics (1047-1058)
     const double v = sol / rho;

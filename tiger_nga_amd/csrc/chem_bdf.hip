@@ -25,7 +25,7 @@ __device__ __forceinline__ int wave_in_block() { return __builtin_amdgcn_readfir
 // finitechem_source_chemistry on a Fortran-layout block.  Persistent: each
 // wave pulls cells from a device counter until all are taken, so that cells
 // whose step counts differ by 10-100x keep every wave busy.
-template <int NEQ>
+template <int NEQ, bool AJ>
 __global__ __launch_bounds__(kBlock) void step_kernel(StepParams p) {
   constexpr TabLayout L = tab_layout<NEQ>();
   __shared__ __attribute__((aligned(16))) char tab_s[L.bytes];
@@ -45,7 +45,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepParams p) {
     if (idx >= ntake) break;
     const int64_t cell = p.cell_list ? (int64_t)p.cell_list[idx] : idx;
     int cnt[TGR_NCOUNT];
-    if (!process_cell<NEQ>(p, m, (tchar*)tab_s, (TGR_LDS RhsLds<NEQ>*)&rl[w], cell,
+    if (!process_cell<NEQ, AJ>(p, m, (tchar*)tab_s, (TGR_LDS RhsLds<NEQ>*)&rl[w], cell,
                            *(TGR_LDS double(*)[kQmaxAbs + 2][kWave]) & zn_s[w], ws, cnt))
       continue;
     if (lane == 0) {
@@ -199,10 +199,10 @@ extern "C" int tgr_step_occupancy(int neq) {
   int nb = 0;
   hipError_t e = hipErrorInvalidValue;
   switch (neq) {
-    case 10: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_kernel<10>, kBlock, 0); break;
-    case 37: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_kernel<37>, kBlock, 0); break;
-    case 40: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_kernel<40>, kBlock, 0); break;
-    case 46: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_kernel<46>, kBlock, 0); break;
+    case 10: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_kernel<10, false>, kBlock, 0); break;
+    case 37: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_kernel<37, false>, kBlock, 0); break;
+    case 40: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_kernel<40, false>, kBlock, 0); break;
+    case 46: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_kernel<46, false>, kBlock, 0); break;
     default: break;
   }
   return e == hipSuccess ? nb : 0;
@@ -212,10 +212,22 @@ extern "C" int tgr_launch_step(const StepParams* p, int neq, int grid, hipStream
   if (!p->queue || !p->ws) return -1;
   hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, p->queue, p->totals);
   switch (neq) {
-    case 10: hipLaunchKernelGGL(step_kernel<10>, dim3(grid), dim3(kBlock), 0, s, *p); break;
-    case 37: hipLaunchKernelGGL(step_kernel<37>, dim3(grid), dim3(kBlock), 0, s, *p); break;
-    case 40: hipLaunchKernelGGL(step_kernel<40>, dim3(grid), dim3(kBlock), 0, s, *p); break;
-    case 46: hipLaunchKernelGGL(step_kernel<46>, dim3(grid), dim3(kBlock), 0, s, *p); break;
+    case 10:
+      if (p->jac) hipLaunchKernelGGL((step_kernel<10, true>), dim3(grid), dim3(kBlock), 0, s, *p);
+      else hipLaunchKernelGGL((step_kernel<10, false>), dim3(grid), dim3(kBlock), 0, s, *p);
+      break;
+    case 37:
+      if (p->jac) hipLaunchKernelGGL((step_kernel<37, true>), dim3(grid), dim3(kBlock), 0, s, *p);
+      else hipLaunchKernelGGL((step_kernel<37, false>), dim3(grid), dim3(kBlock), 0, s, *p);
+      break;
+    case 40:
+      if (p->jac) hipLaunchKernelGGL((step_kernel<40, true>), dim3(grid), dim3(kBlock), 0, s, *p);
+      else hipLaunchKernelGGL((step_kernel<40, false>), dim3(grid), dim3(kBlock), 0, s, *p);
+      break;
+    case 46:
+      if (p->jac) hipLaunchKernelGGL((step_kernel<46, true>), dim3(grid), dim3(kBlock), 0, s, *p);
+      else hipLaunchKernelGGL((step_kernel<46, false>), dim3(grid), dim3(kBlock), 0, s, *p);
+      break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;

@@ -73,6 +73,7 @@ struct tgr_chem {
   int nsp = 0, neq = 0;
   double rtol = 1e-6, atol = 1e-12;
   int max_steps = 10000, qmax = 5;
+  int jac = 0;  // 0: difference-quotient Jacobian (the reference's CVODE FCVDENSE); 1: analytic
   int rhs_lanes = 1;  // batched right-hand side: one state per lane (1) or per wave (0)
   int64_t grid_override = 0;
   int device = 0;
@@ -181,6 +182,7 @@ StepParams step_params(const tgr_chem* c, double dt, double p_thermo, double p_t
   p.atol = c->atol;
   p.max_steps = c->max_steps;
   p.qmax = c->qmax;
+  p.jac = c->jac;
   p.nx = nx;
   p.ny = ny;
   p.nz = nz;
@@ -370,6 +372,9 @@ int tgr_chem_set_option(tgr_chem* c, const char* key, int64_t v) {
   else if (!strcmp(key, "rhs_mapping")) {
     if (v != 0 && v != 1) return fail(TGR_E_ARG, "rhs_mapping must be 0 (wave per state) or 1 (lane per state)");
     c->rhs_lanes = (int)v;
+  } else if (!strcmp(key, "jacobian")) {
+    if (v != 0 && v != 1) return fail(TGR_E_ARG, "jacobian must be 0 (difference quotients) or 1 (analytic)");
+    c->jac = (int)v;
   } else if (!strcmp(key, "max_order")) {
     if (v < 1 || v > 5) return fail(TGR_E_ARG, "max_order must be 1..5");
     c->qmax = (int)v;

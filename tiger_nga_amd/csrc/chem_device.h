@@ -412,6 +412,241 @@ __device__ __forceinline__ double rhs_wave_T(tchar* tab, TGR_LDS RhsLds<NEQ>& S,
   return sp ? rk : (lane == nsp ? tmp / cpm : 0.0);
 }
 
+// ---- analytic Jacobian ----------------------------------------------------
+// The reference has no Jacobian routine (CVODE differences the right-hand
+// side, finitechem.f90:1028).  This is the exact derivative of the same
+// rate expressions, evaluated from the same tables: per reaction lane and
+// round, the coefficients that one column of dCDOT/dC needs are kept in
+// registers, so that each species column is a pass over the reaction rounds
+// (no exp / log) plus the deterministic CDOT gather.
+//   q_r = (k_f P_f - k_b P_b) mm,  P = product of the concentration factors,
+//   mm = M_r for a third-body reaction, else 1;
+//   dq_r/dC_j = mm (k_f dP_f/dC_j - k_b dP_b/dC_j) + g_r eff(m_r, j)
+// with g_r = k_f P_f - k_b P_b for a third-body reaction and, for a fall-off
+// reaction whose third-body sum is positive (conc = M_r, GETLINDRATECOEFF
+// HYDROGENF.f90:295-299), g_r = dk_f/dconc P_f - dk_b/dconc P_b.
+
+// NASA-7 h, cp and dcp/dT of the species in this lane (COMPTHERMODATA forms);
+// lanes >= nsp return 0.
+template <int NEQ>
+__device__ __forceinline__ void thermo_lane_d(tchar* tab, double tmid, double T, int lane, double& h, double& cp,
+                                              double& dcp) {
+  constexpr TabLayout L = tab_layout<NEQ>();
+  tchar* c = tab + L.th + (T > tmid ? 0 : 13 * kRow) + lane * 8;
+  const double a = ld<double>(c, 0), c1 = ld<double>(c, kRow), c2 = ld<double>(c, 2 * kRow),
+               c3 = ld<double>(c, 3 * kRow), c4 = ld<double>(c, 4 * kRow), c5 = ld<double>(c, 5 * kRow),
+               c6 = ld<double>(c, 6 * kRow);
+  const double ac = ld<double>(c, 7 * kRow), d1 = ld<double>(c, 8 * kRow), d2 = ld<double>(c, 9 * kRow),
+               d3 = ld<double>(c, 10 * kRow), d4 = ld<double>(c, 11 * kRow), d5 = ld<double>(c, 12 * kRow);
+  h = a * (T * (c1 + T * (c2 + T * (c3 + T * (c4 + T * c5)))) + c6);
+  cp = ac * (d1 + T * (d2 + T * (d3 + T * (d4 + T * d5))));
+  dcp = ac * (d2 + T * (2.0 * d3 + T * (3.0 * d4 + T * (4.0 * d5))));
+  if (lane >= NEQ - 1) h = cp = dcp = 0.0;
+}
+
+// Troe rate (as troe()) with its derivatives: dk_dT at fixed conc through
+// k_inf, k_0, Fc (and conc = P/RT when dlconc = -1/T), dk_dconc at fixed T.
+// dl*: logarithmic derivatives d ln(.)/dT.
+__device__ __forceinline__ double troe_d(double kinf, double dlkinf, double k0, double dlk0, double conc,
+                                         double dlconc, double lnfc, double dlfc, double ntmp, double cco,
+                                         double& dk_dT, double& dk_dconc) {
+  const double kin = fmax(kinf, 1.0e-60);
+  const double pr = k0 * conc / kin;
+  const double lg = fma(flog(pr), kInvLn10, cco);
+  const double D = fma(-kTroeD, lg, ntmp);
+  const double x = lg / D;
+  const double x2 = fma(x, x, 1.0);
+  const double F = fexp(lnfc / x2);
+  const double k = kinf * F * pr / (1.0 + pr);  // as troe(), bit for bit
+  const double ix2 = 1.0 / x2;
+  const double ipr1 = 1.0 / (1.0 + pr);
+  const double dlpr = dlk0 + dlconc - (kinf > 1.0e-60 ? dlkinf : 0.0);
+  const double dL = (dlpr - kTroeC1 * dlfc) * kInvLn10;
+  const double dn = -kTroeN1 * dlfc * kInvLn10;
+  const double iD2 = 1.0 / (D * D);
+  const double dx = (ntmp * dL - lg * dn) * iD2;
+  const double c2x = lnfc * 2.0 * x * ix2 * ix2;
+  dk_dT = k * (dlkinf + dlfc * ix2 - c2x * dx + dlpr * ipr1);
+  const double dlFp = -c2x * ntmp * kInvLn10 * iD2;
+  dk_dconc = kinf * F * (dlFp * ipr1 + ipr1 * ipr1) * (k0 / kin);
+  return k;
+}
+
+// What one reaction lane keeps for the Jacobian columns: the products of two
+// of its three concentration factors times mm k (forward a**, backward b**),
+// the third-body coefficient g and the packed factor slots; md: the slot of
+// the third-body sum g multiplies (kOne: none).
+struct JacLane {
+  double a01, a02, a12, b01, b02, b12, g;
+  int i0, i1, md;
+};
+
+// Production rates (same values as prodrates_wave) plus dCDOT/dT at fixed C
+// in dcdT, filling jl for jac_column.  ctot: total concentration.
+template <int NEQ>
+__device__ __forceinline__ double prodrates_jac_base(tchar* tab, TGR_LDS RhsLds<NEQ>& S, double Ck, double ctot,
+                                                     double T, double P, JacLane (&jl)[(MechT<NEQ>::nrxn + 63) / 64],
+                                                     double& dcdT) {
+  using MT = MechT<NEQ>;
+  constexpr TabLayout L = tab_layout<NEQ>();
+  const int lane = lane_id();
+  const double lt = flog(T);
+  const double rt = kRgasF4 * T;
+  const double irt = 1.0 / rt;
+  const double invT = 1.0 / T;
+  S.C[lane] = lane == kOne ? 1.0 : Ck;
+  wave_sync();
+  {
+    double s = ctot;
+#pragma unroll
+    for (int e = 0; e < MT::maxe; ++e)
+      s = fma(ld<double>(tab, L.ev + e * kRow + lane * 8), S.C[ld<int>(tab, L.ek + e * kIRow + lane * 4)], s);
+    S.M[lane] = lane == kOne ? 1.0 : (lane == kZero ? 0.0 : s);
+  }
+  wave_sync();
+  double qT[L.rounds];
+#pragma unroll
+  for (int rd = 0; rd < L.rounds; ++rd) {
+    tchar* rx = tab + L.rx + rd * kRxRound + lane * 8;
+    tchar* rxi = tab + L.rx + rd * kRxRound + 6 * kRow + lane * 4;
+    const int i0 = ld<int>(rxi, 0), i1 = ld<int>(rxi, kIRow);
+    const double bf = ld<double>(rx, kRow), ef = ld<double>(rx, 2 * kRow);
+    const double bb = ld<double>(rx, 4 * kRow), eb = ld<double>(rx, 5 * kRow);
+    double kf = arrhenius(ld<double>(rx, 0), bf, ef, lt, irt);
+    double kb = arrhenius(ld<double>(rx, 3 * kRow), bb, eb, lt, irt);
+    // d ln k / dT of the Arrhenius forms: (b - e/(R T)) / T
+    double dkf = kf * (fma(-ef, irt, bf) * invT), dkb = kb * (fma(-eb, irt, bb) * invT);
+    double dkfc = 0.0, dkbc = 0.0;
+    int md = (i0 >> 24) & 255;
+    if (rd == 0 && ((i1 >> 24) & 1)) {
+      tchar* fo = tab + L.fo + lane * 8;
+      const int fi = ld<int>(tab, L.fo + 12 * kRow + lane * 4);
+      const double mv = S.M[(fi >> 8) & 255];
+      const bool usem = mv > 0.0;
+      const double conc = usem ? mv : P / rt;
+      const double dlconc = usem ? 0.0 : -invT;
+      double fc = 0.0, dfc = 0.0;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const bool inv = (fi >> i) & 1;
+        const double gi = ld<double>(fo, (9 + i) * kRow);
+        const double x = -(gi * (inv ? invT : T));
+        const double ci = ld<double>(fo, (6 + i) * kRow), ex = fexp(x);
+        fc = fma(ci, ex, fc);
+        dfc = fma(ci * ex, inv ? gi * invT * invT : -gi, dfc);
+      }
+      const double lnfc = flog(fc);
+      const double lfc = lnfc * kInvLn10;
+      const double ntmp = fma(-kTroeN1, lfc, kTroeN0);
+      const double cco = fma(-kTroeC1, lfc, -kTroeC0);
+      const double dlfc = dfc / fc;
+      const double bf0 = ld<double>(fo, kRow), ef0 = ld<double>(fo, 2 * kRow);
+      const double k0f = arrhenius(ld<double>(fo, 0), bf0, ef0, lt, irt);
+      const double dlinf = fma(-ef, irt, bf) * invT;
+      kf = troe_d(kf, dlinf, k0f, fma(-ef0, irt, bf0) * invT, conc, dlconc, lnfc, dlfc, ntmp, cco, dkf, dkfc);
+      if ((i1 >> 25) & 1) {
+        const double bb0 = ld<double>(fo, 4 * kRow), eb0 = ld<double>(fo, 5 * kRow);
+        const double k0b = arrhenius(ld<double>(fo, 3 * kRow), bb0, eb0, lt, irt);
+        kb = troe_d(kb, fma(-eb, irt, bb) * invT, k0b, fma(-eb0, irt, bb0) * invT, conc, dlconc, lnfc, dlfc, ntmp,
+                    cco, dkb, dkbc);
+      }
+      md = usem ? ((fi >> 8) & 255) : kZero;
+    }
+    const double c0 = S.C[i0 & 255], c1 = S.C[(i0 >> 8) & 255], c2 = S.C[(i0 >> 16) & 255];
+    const double p0 = S.C[i1 & 255], p1 = S.C[(i1 >> 8) & 255], p2 = S.C[(i1 >> 16) & 255];
+    const double mm = S.M[(i0 >> 24) & 255];
+    const double pf = c0 * c1 * c2, pb = p0 * p1 * p2;
+    const double wf = kf * pf, wb = kb * pb;
+    S.q[rd * 64 + lane] = (wf - wb) * mm;
+    qT[rd] = (dkf * pf - dkb * pb) * mm;
+    const double af = kf * mm, ab = kb * mm;
+    JacLane& J = jl[rd];
+    J.a01 = af * (c0 * c1);
+    J.a02 = af * (c0 * c2);
+    J.a12 = af * (c1 * c2);
+    J.b01 = ab * (p0 * p1);
+    J.b02 = ab * (p0 * p2);
+    J.b12 = ab * (p1 * p2);
+    // third-body reactions (mm = M_r): d/dM of (wf - wb) M is wf - wb; fall-off: d/dconc
+    J.g = (rd == 0 && ((i1 >> 24) & 1)) ? dkfc * pf - dkbc * pb : wf - wb;
+    J.i0 = i0;
+    J.i1 = i1;
+    J.md = md;
+  }
+  wave_sync();
+  double cdot = 0.0, dT = 0.0;
+  // gather CDOT, then dCDOT/dT from the same schedule
+#pragma unroll
+  for (int pass = 0; pass < 2; ++pass) {
+    if (pass == 1) {
+#pragma unroll
+      for (int rd = 0; rd < L.rounds; ++rd) S.q[rd * 64 + lane] = qT[rd];
+      wave_sync();
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < MT::gmax; ++i)
+      acc = fma(ld<double>(tab, L.gn + i * kRow + lane * 8), S.q[ld<int>(tab, L.gr + i * kIRow + lane * 4)], acc);
+    S.part[lane] = acc;
+    wave_sync();
+    double v = 0.0;
+    if (lane < NEQ - 1) {
+      const int f0 = ld<int>(tab, L.sf + lane * 4), nn = ld<int>(tab, L.sn + lane * 4);
+      for (int i = 0; i < nn; ++i) v += S.part[f0 + i];
+    }
+    wave_sync();
+    if (pass == 0) cdot = v;
+    else dT = v;
+  }
+  dcdT = dT;
+  return cdot;
+}
+
+// Column j (species, wave-uniform) of dCDOT/dC: lane k returns
+// d CDOT_k / d C_j.  S.C must hold the concentrations of the base pass;
+// S.M is overwritten with the efficiencies of species j.
+template <int NEQ>
+__device__ __forceinline__ double jac_column(tchar* tab, TGR_LDS RhsLds<NEQ>& S,
+                                             const JacLane (&jl)[(MechT<NEQ>::nrxn + 63) / 64], int j) {
+  using MT = MechT<NEQ>;
+  constexpr TabLayout L = tab_layout<NEQ>();
+  const int lane = lane_id();
+  {
+    // efficiency of species j in third-body sum `lane`: 1 plus its correction
+    double e = 1.0;
+#pragma unroll
+    for (int q = 0; q < MT::maxe; ++q)
+      e += ld<int>(tab, L.ek + q * kIRow + lane * 4) == j ? ld<double>(tab, L.ev + q * kRow + lane * 8) : 0.0;
+    S.M[lane] = lane < MT::nthird ? e : 0.0;
+  }
+  wave_sync();
+#pragma unroll
+  for (int rd = 0; rd < L.rounds; ++rd) {
+    const JacLane& J = jl[rd];
+    double d = J.g * S.M[J.md];
+    d += ((J.i0 & 255) == j) ? J.a12 : 0.0;
+    d += (((J.i0 >> 8) & 255) == j) ? J.a02 : 0.0;
+    d += (((J.i0 >> 16) & 255) == j) ? J.a01 : 0.0;
+    d -= ((J.i1 & 255) == j) ? J.b12 : 0.0;
+    d -= (((J.i1 >> 8) & 255) == j) ? J.b02 : 0.0;
+    d -= (((J.i1 >> 16) & 255) == j) ? J.b01 : 0.0;
+    S.q[rd * 64 + lane] = d;
+  }
+  wave_sync();
+  double acc = 0.0;
+#pragma unroll
+  for (int i = 0; i < MT::gmax; ++i)
+    acc = fma(ld<double>(tab, L.gn + i * kRow + lane * 8), S.q[ld<int>(tab, L.gr + i * kIRow + lane * 4)], acc);
+  S.part[lane] = acc;
+  wave_sync();
+  double v = 0.0;
+  if (lane < NEQ - 1) {
+    const int f0 = ld<int>(tab, L.sf + lane * 4), nn = ld<int>(tab, L.sn + lane * 4);
+    for (int i = 0; i < nn; ++i) v += S.part[f0 + i];
+  }
+  return v;
+}
+
 template <int NEQ, int MODE>
 __device__ __attribute__((noinline)) double rhs_call_T(tchar* tab, TGR_LDS RhsLds<NEQ>* S, TGR_LDS RateCache<NEQ>* rc,
                                                        double tmid, double y, double p_gp, double T) {

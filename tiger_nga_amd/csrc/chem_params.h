@@ -27,6 +27,7 @@ struct StepParams {
   DevMech m;
   double dt, p_gp, rtol, atol;
   int max_steps, qmax;
+  int jac;              // 0: difference-quotient Jacobian (CVODE's, as the reference); 1: analytic
   int nx, ny, nz;
   int64_t sx, sy, sz, base, plane;
   int isc_sc0, isc_T0;  // 0-based scalar planes of the first species and of T

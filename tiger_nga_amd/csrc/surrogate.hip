@@ -89,7 +89,7 @@ __global__ void surr_reset_kernel(unsigned int* queue, unsigned int* rej_count, 
 // AD: the error-controlled schedule; NET: the stages start from the
 // network's guesses (one kernel instance per combination keeps the other
 // paths' code out of the register allocation)
-template <int NEQ, bool AD, bool NET>
+template <int NEQ, bool AD, bool NET, bool AJ>
 __device__ __forceinline__ int surr_cell(const SurrParams& P, const DevMech& m, tchar* tab, TGR_LDS RhsLds<NEQ>* rl,
                                          TGR_LDS RateCache<NEQ>* rc, int64_t cell, double* ws,
                                          int& nit, int& nfe, int& nlu) {
@@ -114,8 +114,8 @@ __device__ __forceinline__ int surr_cell(const SurrParams& P, const DevMech& m, 
     return sol0 + rho * osc * (double)gcell[(int64_t)(i * kSurrStages + j) * P.guess_stride];
   };
   const double wacc = lane < nsp ? 1.0 / (P.tol_y * rho) : (lane == nsp ? 1.0 / (P.tol_t * rho) : 0.0);
-  Bdf<NEQ> b(m, tab, rl, ws, p.p_gp, p.rtol, p.atol);
-  if (P.split_jac) b.rc = rc;
+  Bdf<NEQ, AJ> b(m, tab, rl, ws, p.p_gp, p.rtol, p.atol);
+  if (P.split_jac && !AJ) b.rc = rc;
   int st;
   if constexpr (AD) {
     // the network's state at time t: linear in t between its stage times
@@ -246,7 +246,7 @@ __device__ __forceinline__ int surr_cell(const SurrParams& P, const DevMech& m, 
   return 0;
 }
 
-template <int NEQ, bool AD, bool NET>
+template <int NEQ, bool AD, bool NET, bool AJ>
 __global__ __launch_bounds__(kSurrBlock) void surr_kernel(SurrParams P) {
   constexpr TabLayout L = tab_layout<NEQ>();
   __shared__ __attribute__((aligned(16))) char tab_s[L.bytes];
@@ -264,8 +264,8 @@ __global__ __launch_bounds__(kSurrBlock) void surr_kernel(SurrParams P) {
     const int64_t cell = (int64_t)(unsigned int)__builtin_amdgcn_readfirstlane((int)c);
     if (cell >= P.s.ncell) break;
     int nit = 0, nfe = 0, nlu = 0;
-    const int st = surr_cell<NEQ, AD, NET>(P, m, (tchar*)tab_s, (TGR_LDS RhsLds<NEQ>*)&rl[w],
-                                  (TGR_LDS RateCache<NEQ>*)&rcs[w], cell, ws, nit, nfe, nlu);
+    const int st = surr_cell<NEQ, AD, NET, AJ>(P, m, (tchar*)tab_s, (TGR_LDS RhsLds<NEQ>*)&rl[w],
+                                               (TGR_LDS RateCache<NEQ>*)&rcs[w], cell, ws, nit, nfe, nlu);
     if (st < 0) continue;
     if (lane == 0) {
       if (st != 0) {
@@ -290,8 +290,8 @@ extern "C" int tgr_surr_block_waves(void) { return kSurrWaves; }
 template <int NEQ>
 static hipError_t surr_occ(int* nb, bool adaptive) {
   // the instances differ in registers only a little; the default one decides
-  return adaptive ? hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, surr_kernel<NEQ, true, false>, kSurrBlock, 0)
-                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, surr_kernel<NEQ, false, false>, kSurrBlock, 0);
+  return adaptive ? hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, surr_kernel<NEQ, true, false, false>, kSurrBlock, 0)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, surr_kernel<NEQ, false, false, false>, kSurrBlock, 0);
 }
 
 extern "C" int tgr_surr_occupancy(int neq, int adaptive) {
@@ -328,17 +328,23 @@ extern "C" int tgr_launch_surr_set_column(uint16_t* X, int64_t rows, int in_pad,
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-template <int NEQ>
-static void launch_surr(const SurrParams& P, bool adaptive, int grid, hipStream_t s) {
+template <int NEQ, bool AJ>
+static void launch_surr_j(const SurrParams& P, bool adaptive, int grid, hipStream_t s) {
   const bool net = P.use_guess != 0;
   if (adaptive && net)
-    surr_kernel<NEQ, true, true><<<dim3(grid), dim3(kSurrBlock), 0, s>>>(P);
+    surr_kernel<NEQ, true, true, AJ><<<dim3(grid), dim3(kSurrBlock), 0, s>>>(P);
   else if (adaptive)
-    surr_kernel<NEQ, true, false><<<dim3(grid), dim3(kSurrBlock), 0, s>>>(P);
+    surr_kernel<NEQ, true, false, AJ><<<dim3(grid), dim3(kSurrBlock), 0, s>>>(P);
   else if (net)
-    surr_kernel<NEQ, false, true><<<dim3(grid), dim3(kSurrBlock), 0, s>>>(P);
+    surr_kernel<NEQ, false, true, AJ><<<dim3(grid), dim3(kSurrBlock), 0, s>>>(P);
   else
-    surr_kernel<NEQ, false, false><<<dim3(grid), dim3(kSurrBlock), 0, s>>>(P);
+    surr_kernel<NEQ, false, false, AJ><<<dim3(grid), dim3(kSurrBlock), 0, s>>>(P);
+}
+
+template <int NEQ>
+static void launch_surr(const SurrParams& P, bool adaptive, int grid, hipStream_t s) {
+  if (P.s.jac) launch_surr_j<NEQ, true>(P, adaptive, grid, s);
+  else launch_surr_j<NEQ, false>(P, adaptive, grid, s);
 }
 
 extern "C" int tgr_launch_surr(const SurrParams* P, int neq, int grid, hipStream_t s) {
