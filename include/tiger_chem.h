@@ -97,6 +97,24 @@ int tgr_chem_source_chemistry_device(tgr_chem *ctx, double dt, double p_thermo, 
                                      const tgr_grid *g, double *RHO_, double *SC_, const int *mask,
                                      double *HR_gp, double *chemsrc_gp, void *stream, tgr_stats *stats);
 
+/* The same step on DEVICE arrays, enqueued on `stream`, allocating nothing.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory during the call (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The work counter, the counter totals and
+ * the per-wave Jacobian workspace come from a 48 MiB file-scope device array
+ * (one per GPU, held until the process ends) instead of context buffers; the
+ * call makes no hipMalloc / hipFree and no host-device copy, returns after
+ * enqueueing two kernels and reports no statistics.  It returns TGR_E_ARG when
+ * the persistent grid's workspace (workgroups x 8 waves x neq x 512 bytes)
+ * exceeds 48 MiB - 256 bytes: the default grid on MI355X fits every compiled
+ * mechanism (EthyRedF needs 46 MiB); a larger "grid" option may not.  Every
+ * call on a GPU shares that GPU's array: calls that overlap on one GPU (two
+ * streams, two contexts, two host threads) race and must be serialised by the
+ * caller.  The counters are cleared by a kernel at the start of each call. */
+int tgr_chem_source_chemistry_device_noalloc(tgr_chem *ctx, double dt, double p_thermo, double p_thermo_old,
+                                             const tgr_grid *g, double *RHO_, double *SC_, const int *mask,
+                                             double *HR_gp, double *chemsrc_gp, void *stream);
+
 /* PRODRATES (HYDROGENF.f90:4-277) for n states, device arrays, SoA:
  * C[nsp][n] kmol/m3, T[n] K, P[n] Pa -> cdot[nsp][n] kmol/m3/s. */
 int tgr_chem_prodrates_device(tgr_chem *ctx, int64_t n, const double *C, const double *T, const double *P,

@@ -12,7 +12,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libtiger_chem.so")
-SOURCES = ["chem_bdf.hip", "props.hip", "surrogate.hip", "mlp.hip", "chem_capi.cpp", "tables_host.cpp"]
+SOURCES = ["chem_bdf.hip", "props.hip", "surrogate.hip", "mlp.hip", "noalloc.hip", "chem_capi.cpp",
+           "tables_host.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TGR_OFFLOAD_ARCH", "gfx950")
 

@@ -232,10 +232,18 @@ int persistent_grid(const tgr_chem* c, int occupancy, int64_t ncell, int wpb) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(gs, (ncell + wpb - 1) / wpb));
 }
 
-// the plain step; with list / nlist only the listed cells (device list and count)
+// Layout of an external scratch block (tgr_chem_source_chemistry_device_noalloc):
+// the work counter at byte 0, the counter totals at byte 64, the per-wave
+// Jacobian workspace from byte 256.
+constexpr size_t kScrTotals = 64, kScrWs = 256;
+
+// the plain step; with list / nlist only the listed cells (device list and
+// count); with scratch (and stats == NULL) the counters and workspace come from
+// that block of scratch_bytes instead of the context's buffers
 int run_step(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const tgr_grid* g, double* rho,
              double* sc, int isc_sc0, int isc_T0, const int* mask, double* hr, double* chem, hipStream_t s,
-             tgr_stats* stats, const int* list = nullptr, const unsigned int* nlist = nullptr) {
+             tgr_stats* stats, const int* list = nullptr, const unsigned int* nlist = nullptr,
+             void* scratch = nullptr, size_t scratch_bytes = 0) {
   if (!(dt > 0.0) || !std::isfinite(dt)) return fail(TGR_E_ARG, "dt must be positive and finite");
   const int nx = g->imax_ - g->imin_ + 1, ny = g->jmax_ - g->jmin_ + 1, nz = g->kmax_ - g->kmin_ + 1;
   if (stats) memset(stats, 0, sizeof(*stats));
@@ -245,10 +253,22 @@ int run_step(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const
   p.nlist = nlist;
   const int grid = persistent_grid(c, tgr_step_occupancy(c->neq), p.ncell, tgr_step_block_waves());
   const int wpb = tgr_step_block_waves();
-  int rc0 = c->b_ws.ensure((size_t)grid * wpb * c->neq * 64 * sizeof(double));
-  if (rc0) return rc0;
-  p.queue = c->d_queue;
-  p.ws = (double*)c->b_ws.p;
+  const size_t ws_bytes = (size_t)grid * wpb * c->neq * 64 * sizeof(double);
+  if (scratch) {
+    if (ws_bytes > scratch_bytes - kScrWs)
+      return fail(TGR_E_ARG, "the step's workspace (" + std::to_string(ws_bytes) + " bytes for a grid of " +
+                                 std::to_string(grid) + " workgroups) exceeds the " +
+                                 std::to_string(scratch_bytes - kScrWs) +
+                                 "-byte device scratch of the non-allocating entry point; set a smaller \"grid\"");
+    p.queue = (unsigned int*)scratch;
+    p.totals = (unsigned long long*)((char*)scratch + kScrTotals);
+    p.ws = (double*)((char*)scratch + kScrWs);
+  } else {
+    int rc0 = c->b_ws.ensure(ws_bytes);
+    if (rc0) return rc0;
+    p.queue = c->d_queue;
+    p.ws = (double*)c->b_ws.p;
+  }
   // with stats == NULL the call enqueues two kernels and nothing else (no
   // memset, no event), so that it can be captured in a HIP graph
   if (stats) HIP_TRY(hipEventRecord(c->ev0, s));
@@ -313,6 +333,12 @@ int tgr_chem_create(const char* mech, double rtol, double atol, int max_steps, t
   if (!rc && hipMalloc((void**)&c->d_totals, 256) != hipSuccess) rc = fail(TGR_E_NOMEM, "totals alloc");
   if (!rc && (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess))
     rc = fail(TGR_E_HIP, "event create");
+  // device address of this GPU's scratch for the non-allocating entry point
+  // (resolved here, with the context's device current, once per device)
+  void* scr = nullptr;
+  size_t scr_bytes = 0;
+  if (!rc && tgr_scratch_resolve(c->device, &scr, &scr_bytes) != 0)
+    rc = fail(TGR_E_HIP, "hipGetSymbolAddress of the device scratch failed");
   if (rc) {
     tgr_chem_destroy(c);
     return rc;
@@ -418,6 +444,20 @@ int tgr_chem_source_chemistry_device(tgr_chem* c, double dt, double p_thermo, do
   if (rc) return rc;
   return run_step(c, dt, p_thermo, p_thermo_old, g, RHO_, SC_, g->isc_sc - 1, g->isc_T - 1, mask, HR_gp,
                   chemsrc_gp, (hipStream_t)stream, stats);
+}
+
+int tgr_chem_source_chemistry_device_noalloc(tgr_chem* c, double dt, double p_thermo, double p_thermo_old,
+                                             const tgr_grid* g, double* RHO_, double* SC_, const int* mask,
+                                             double* HR_gp, double* chemsrc_gp, void* stream) {
+  if (!c || !RHO_ || !SC_) return fail(TGR_E_ARG, "NULL argument");
+  int rc = check_grid(g, c->nsp);
+  if (rc) return rc;
+  void* scr = nullptr;
+  size_t scr_bytes = 0;
+  if (tgr_scratch_resolve(c->device, &scr, &scr_bytes) != 0)
+    return fail(TGR_E_HIP, "device scratch not resolved for the context's device");
+  return run_step(c, dt, p_thermo, p_thermo_old, g, RHO_, SC_, g->isc_sc - 1, g->isc_T - 1, mask, HR_gp,
+                  chemsrc_gp, (hipStream_t)stream, nullptr, nullptr, nullptr, scr, scr_bytes);
 }
 
 int tgr_chem_set_surrogate(tgr_chem* c, tgr_mlp* net, const float* in_shift, const float* in_scale,

@@ -2,6 +2,7 @@
 // kernels (chem_bdf.hip).  Internal to the library.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "chem_params.h"
@@ -24,4 +25,6 @@ int tgr_launch_surr_features(const tgr::StepParams* p, int neq, int64_t rows, in
                              const float* scale, float log_dt, uint16_t* X, hipStream_t s);
 int tgr_launch_surr_set_column(uint16_t* X, int64_t rows, int in_pad, int col, float value, hipStream_t s);
 int tgr_launch_surr(const tgr::SurrParams* P, int neq, int grid, hipStream_t s);
+// device address and size of the non-allocating entry point's scratch (noalloc.hip)
+int tgr_scratch_resolve(int device, void** ptr, size_t* bytes);
 }

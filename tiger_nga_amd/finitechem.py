@@ -138,6 +138,7 @@ def load_library(path: str = LIB_PATH):
     step_args = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.POINTER(_Grid), vp, vp, vp, vp, vp]
     lib.tgr_chem_source_chemistry.argtypes = step_args + [ctypes.POINTER(_Stats)]
     lib.tgr_chem_source_chemistry_device.argtypes = step_args + [vp, ctypes.POINTER(_Stats)]
+    lib.tgr_chem_source_chemistry_device_noalloc.argtypes = step_args + [vp]
     lib.tgr_chem_prodrates_device.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp, vp]
     lib.tgr_chem_rhs_device.argtypes = [vp, ctypes.c_int64, ctypes.c_double, vp, vp, vp]
     lib.tgr_chem_temperature_device.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp, vp, vp]
