@@ -291,6 +291,45 @@ def bench_surrogate(a, torch, fc, g, R_d, S0_d, S_d, HR_d, CS_d, dt, P, mech, sp
                      "the network's guesses" if guess_t else "the derivative predictor")}
 
 
+def bench_standalone(a, torch, mech, g, R_d, S0_d, S_d, dt, P, sp, cells):
+    """The standalone surrogate (surr_mode 1) on the same block: the network's
+    predictions at the stage times of 4 equal sub-steps, each stage checked by
+    one FP64 Newton correction, the embedded error estimate, element projection
+    and enthalpy check; rejected cells by the plain FP64 step.  With the
+    network and with the zero-change prediction (the method without it); a
+    context of its own, so the guided path's options stay as tuned."""
+    from tiger_nga_amd import autotune, surrogate
+    from tiger_nga_amd.finitechem import FiniteChem
+    path = surrogate.model_path(mech)
+    side = max(1, min(a.steps, a.side_steps))
+    res = {}
+    for guess in (1, 0):
+        f2 = FiniteChem(mech, 1e-6, 1e-12, 10000)
+        autotune.apply(f2, cells)
+        surrogate.Surrogate.load(path).attach(f2)
+        for k, v in (("surr_probe", 0), ("surr_mode", 1), ("surr_nsub", 4), ("surr_grade", 1.0), ("surr_tol_y", 1e-6),
+                     ("surr_guess", guess)):
+            f2.set_option(k, v)
+        _progress("standalone surrogate (%s), %d steps" % ("network" if guess else "zero-change prediction", side))
+        t = _time_steps(torch, lambda: f2.source_chemistry_surrogate(R_d, S_d, g, dt, P, P, stream=sp, sync=False),
+                        S_d, S0_d, side, 1)
+        S_d.copy_(S0_d)
+        _, ss = f2.source_chemistry_surrogate(R_d, S_d, g, dt, P, P, stream=sp)
+        f2.close()
+        res["network" if guess else "zero_change_prediction"] = {
+            "value": cells / t, "ms_per_step": t * 1e3, "steps": side,
+            "accepted_fraction": ss["accepted"] / max(1, cells),
+            "rejected": {"residual": ss["rej_newton"], "error_estimate": ss["rej_error"],
+                         "enthalpy": ss["rej_enthalpy"]},
+            "check_ms": ss["surrogate_ms"], "fallback_ms": ss["fallback_ms"], "_state": S_d.clone()}
+    res["method"] = ("surr_mode 1: per cell, the network's predictions at the 12 stage times of 4 equal SDIRK3(2) "
+                     "sub-steps are the stage values; each stage equation's residual gives one FP64 Newton "
+                     "correction with the frozen Jacobian (accepted when <= 1 acceptance tolerance, 1e-6 in Y, and "
+                     "applied), then the embedded error estimate, element projection and enthalpy check; every "
+                     "rejected cell by the plain FP64 step")
+    return res
+
+
 def bench_properties(a, torch, fc, g, S_d, R_d, P, n, cells_label, mech, sp):
     """Density, viscosity and diffusivities of every cell (finitechem.f90:324-461)."""
     from tiger_nga_amd import flops
@@ -469,8 +508,34 @@ def main():
                                        "max_T_err_K": float(dT[ix].max())} for k, ix in full_cls.items()}
             return res
         out["plain_vs_reference"] = vs_ref(S_plain)
+    # the plain step with the other Jacobian (difference quotients, as the
+    # reference's CVODE, or analytic): side measurement
+    jac_t = int(fc.options.get("jacobian", 0))
+    fc.set_option("jacobian", 1 - jac_t)
+    k_side = max(1, min(a.steps, a.side_steps))
+    _progress("plain step with jacobian %d, %d steps" % (1 - jac_t, k_side))
+    t_oj = _time_steps(torch, lambda: fc.source_chemistry(R_d, S_d, g, dt, P, P, HR_gp=HR_d, chemsrc_gp=CS_d,
+                                                          stream=sp, sync=False), S_d, S0_d, k_side, 1)
+    S_d.copy_(S0_d)
+    st_oj = fc.source_chemistry(R_d, S_d, g, dt, P, P, stream=sp, sync=True)
+    fc.set_option("jacobian", jac_t)
+    out["config"]["jacobian"] = "analytic" if jac_t else "difference quotients (as the reference's CVODE)"
+    out["plain_other_jacobian"] = {
+        "jacobian": "analytic" if not jac_t else "difference quotients",
+        "value": cells / t_oj, "ms_per_step": t_oj * 1e3, "steps": k_side, "kernel_ms": st_oj["kernel_ms"],
+        "solver": {k: st_oj[k] for k in ("nst", "nfe", "nfe_dq", "nje", "nlu", "failed")},
+        "roofline_frac": flops.step_flops(info, st_oj) / (st_oj["kernel_ms"] * 1e-3) / PEAK_FP64}
+    if ref_rows is not None:
+        out["plain_other_jacobian"]["vs_reference"] = vs_ref(S_d)
     sur = bench_surrogate(a, torch, fc, g, R_d, S0_d, S_d, HR_d, CS_d, dt, P, mech, sp, cells) if have_guided \
         else None
+    if have_guided:
+        sa = bench_standalone(a, torch, mech, g, R_d, S0_d, S_d, dt, P, sp, cells)
+        for key in ("network", "zero_change_prediction"):
+            st_sa = sa[key].pop("_state")
+            if ref_rows is not None:
+                sa[key]["vs_reference"] = vs_ref(st_sa)
+        out["standalone_surrogate"] = sa
     sur_state = None
     if sur is not None:
         sur_state = sur.pop("_state")

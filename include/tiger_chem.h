@@ -243,7 +243,12 @@ int tgr_chem_load_surrogate(tgr_chem *ctx, const char *path);
  * (adaptive: largest step growth, 1.5..100, default 10), "surr_keep" (adaptive:
  * growth factors below this keep the step and its LU factors, 1..4, default
  * 1.5; on GRI-3.0 64^3 this halves the LU count and makes the guided step 9 %
- * faster at unchanged error). */
+ * faster at unchanged error), "surr_jac_theta" (adaptive: a step attempt whose
+ * Newton iterations contract slower than this rate, or fail, with a Jacobian
+ * of an earlier state re-evaluates the Jacobian at the current state before
+ * the next attempt, while "surr_max_jac" allows; default 0.3).  Every stage's
+ * Newton iteration stops when max(1, theta / (1 - theta)) |correction| is below
+ * surr_newton_tol acceptance tolerances, theta its contraction rate. */
 int tgr_chem_set_option_real(tgr_chem *ctx, const char *key, double value);
 
 /* Tuning knobs (no counterpart in the reference):
@@ -272,6 +277,22 @@ int tgr_chem_set_option_real(tgr_chem *ctx, const char *key, double value);
  *                    plus df/dT times dT/dy (T = rho T / rho); 0: every column a
  *                    full difference quotient as in the plain step
  *   "surr_newton_max"  Newton iterations per stage before the stage fails (default 8)
+ *   "surr_mode"   0 (default): the guided solve above; 1: standalone surrogate --
+ *                 the network's predictions at the stage times of the fixed
+ *                 schedule (surr_nsub sub-steps of ratio surr_grade) are taken
+ *                 as the stage values, each checked by ONE FP64 Newton
+ *                 correction of its stage equation z = base + gamma h f(z) with
+ *                 the frozen Jacobian (the correction is the prediction's
+ *                 distance from the equation's solution, and is applied); a
+ *                 stage whose correction exceeds "surr_sa_tol" (real, default
+ *                 1) acceptance tolerances, or a sub-step whose embedded error
+ *                 estimate exceeds 1, rejects the cell; accepted cells then pass
+ *                 the element projection and enthalpy check as above, rejected
+ *                 ones go to the plain step.  With "surr_guess" 0 the same
+ *                 check runs on the zero-change prediction (each stage starts
+ *                 from the previous stage value): the method without the network
+ *   "surr_max_jac"   adaptive schedule: Jacobians per cell, 1..64 (default 4; 1: the
+ *                    initial one only; see surr_jac_theta)
  *   "surr_probe"  adaptive use: a guided probe every this many calls while the
  *                 guided solve is not paying (default 8; 0 = always guided) */
 int tgr_chem_set_option(tgr_chem *ctx, const char *key, int64_t value);

@@ -151,3 +151,59 @@ def test_network_guess_adaptive(torch_dev):
         it[guess] = (ss["newton_iters"] / (16 ** 3), ss["accepted"] / (16 ** 3))
     print("\nadaptive schedule, Newton iterations per cell (accepted fraction): with the network %.1f (%.3f), "
           "derivative start %.1f (%.3f)" % (it[1][0], it[1][1], it[0][0], it[0][1]))
+
+
+@pytest.mark.parametrize("mech,workload,relax", [("gri30", "gri30_uneven", 0), ("gri30", "gri30_uneven", 1),
+                                                 ("h2", "h2", 1), ("dme39", "dme39", 1)])
+def test_standalone_surrogate(torch_dev, mech, workload, relax):
+    """surr_mode 1, the standalone surrogate: the network's predictions at the
+    stage times of 4 equal sub-steps are the stage values, each checked by one
+    FP64 Newton correction of its stage equation (accepted within surr_sa_tol
+    = 1 acceptance tolerance), then the embedded error estimate, the element
+    projection and the enthalpy check; the plain FP64 step integrates every
+    other cell.  Every cell ends accepted (within 4e-6 of the tight solution --
+    four sub-steps, each within 1e-6 -- and element-conserving) or bit-identical
+    to the plain step.  The accepted fraction with the network is printed next
+    to the same check on the zero-change prediction (the method without the
+    network)."""
+    torch = torch_dev
+    from tiger_nga_amd import inputs, surrogate
+    from tiger_nga_amd.finitechem import FiniteChem
+    n = 16 ** 3
+    fc = FiniteChem(mech, 1e-6, 1e-12)
+    R_d, S0, g, P = surrogate.block_states(fc, workload, n, 3, relax)
+    dt = inputs.WORKLOADS[workload][3]
+    nsp = fc.nsp
+    Sp = S0.clone()
+    fc.source_chemistry(R_d, Sp, g, dt, P, P)
+    ft = FiniteChem(mech, 1e-10, 1e-16, 200000)
+    St = S0.clone()
+    ft.source_chemistry(R_d, St, g, dt, P, P)
+    ft.close()
+    _model(mech).attach(fc)
+    fc.set_option("surr_probe", 0)
+    fc.set_option("surr_mode", 1)
+    fc.set_option("surr_nsub", 4)
+    fc.set_option("surr_grade", 1.0)
+    fc.set_option("surr_tol_y", 1e-6)
+    acc = {}
+    Et = torch.from_numpy(np.ascontiguousarray((meta.element_matrix(mech) / meta.molar_mass(mech)[None, :]).T)).cuda()
+    el0 = S0[..., :nsp].contiguous() @ Et
+    for guess in (1, 0):
+        fc.set_option("surr_guess", guess)
+        Ss = S0.clone()
+        st, ss = fc.source_chemistry_surrogate(R_d, Ss, g, dt, P, P)
+        assert ss["accepted"] + ss["rejected"] == n == st["cells"] and st["failed"] == 0
+        same = (Ss == Sp).all(dim=3)
+        assert int((~same).sum()) <= ss["accepted"]
+        sel = ~same
+        err = float((Ss[..., :nsp] - St[..., :nsp]).abs().amax(dim=3)[sel].max()) if bool(sel.any()) else 0.0
+        el1 = Ss[..., :nsp].contiguous() @ Et
+        cons = float(((el1 - el0).abs() / el0.abs().clamp_min(1e-30))[sel].max()) if bool(sel.any()) else 0.0
+        acc[guess] = (ss["accepted"] / n, err, ss["rej_newton"], ss["rej_error"], ss["rej_enthalpy"])
+        print("\n%s relax %d standalone, %s: accepted %.4f, max |dY| of accepted cells vs the tight solution %.2e, "
+              "element drift %.1e; rejected: residual %d, error estimate %d, enthalpy %d"
+              % (workload, relax, "network" if guess else "zero-change prediction", acc[guess][0], err, cons,
+                 ss["rej_newton"], ss["rej_error"], ss["rej_enthalpy"]))
+        assert err <= 4e-6 and cons <= 1e-12
+    fc.close()

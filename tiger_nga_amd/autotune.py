@@ -8,9 +8,13 @@ Choices (all exposed as context options of include/tiger_chem.h):
 * ``grid``         workgroups of the persistent step kernel: the resident
   capacity (occupancy x CUs) or a fraction of it (fewer workgroups leave CUs
   idle but reduce contention for the L2 and the work counter at small sizes);
+* ``jacobian``     the plain step's Jacobian: difference quotients (the
+  reference's CVODE) or analytic -- both reach the same solution; the faster
+  one is kept;
 * the surrogate-guided solve (when a trained network exists): schedule
   (``surr_adaptive``, ``surr_nsub``, ``surr_grade``, ``surr_max_att``,
-  ``surr_h0_fac``),
+  ``surr_h0_fac``), Jacobian updates on slow Newton contraction
+  (``surr_max_jac`` 1 or 4 at ``surr_jac_theta`` 0.3),
   network use (``surr_guess``) and acceptance threshold (``surr_tol_y``
   2.5e-7 .. 2e-6): the fastest combination, timed on freshly generated plus relaxed
   states (the block after one plain step), whose max species error against
@@ -49,6 +53,10 @@ SURR_SCHEDULES = ((1, 8, 2.0, 48), (1, 4, 1.0, 32), (1, 8, 3.0, 64), (0, 4, 1.0,
 SURR_TOLS = (2.5e-7, 5e-7, 1e-6, 2e-6)
 # first step of the adaptive schedule from the initial rate (0: the schedule's)
 SURR_H0 = (0.0, 4.0, 16.0)
+# Jacobians per cell of the adaptive schedule (updates when Newton contracts
+# slower than SURR_JAC_THETA with a Jacobian of an earlier state)
+SURR_MAX_JAC = (1, 4)
+SURR_JAC_THETA = 0.3
 
 
 def _device_name():
@@ -116,6 +124,16 @@ def tune(mech: str, cells: int = 32 ** 3, seed: int = 7, reps: int = 2, log=prin
     rec["grid"] = best_grid
     rec["measured"]["step_ms"] = {str(k): v * 1e3 for k, v in gt.items()}
     fc.set_option("grid", best_grid)
+    # the plain step's Jacobian: difference quotients or analytic
+    jt = {}
+    for jac in (0, 1):
+        fc.set_option("jacobian", jac)
+        jt[jac] = _time(torch, step, reps)
+        log("  step jacobian %s: %.2f ms" % ("analytic" if jac else "difference quotients", jt[jac] * 1e3))
+    best_jac = 1 if jt[1] < 0.98 * jt[0] else 0
+    rec["jacobian"] = best_jac
+    rec["measured"]["step_jacobian_ms"] = {"difference_quotients": jt[0] * 1e3, "analytic": jt[1] * 1e3}
+    fc.set_option("jacobian", best_jac)
     # surrogate-guided solve: schedule, network use and acceptance threshold,
     # measured on freshly generated states and on relaxed ones (the block
     # after one plain step, as in a running simulation)
@@ -152,9 +170,12 @@ def tune(mech: str, cells: int = 32 ** 3, seed: int = 7, reps: int = 2, log=prin
                 # without the network (the network's first-step guess assumes
                 # the schedule's first sub-step)
                 for h0f in (SURR_H0 if adaptive and not guess else (0.0,)):
-                    combos.append((adaptive, nsub, grade, att, guess, h0f))
-        for (adaptive, nsub, grade, att, guess, h0f) in combos:
+                    for mj in (SURR_MAX_JAC if adaptive else (1,)):
+                        combos.append((adaptive, nsub, grade, att, guess, h0f, mj))
+        fc.set_option("surr_jac_theta", SURR_JAC_THETA)
+        for (adaptive, nsub, grade, att, guess, h0f, mj) in combos:
             for tol in SURR_TOLS:
+                fc.set_option("surr_max_jac", mj)
                 fc.set_option("surr_adaptive", adaptive)
                 fc.set_option("surr_nsub", nsub)
                 fc.set_option("surr_grade", grade)
@@ -174,19 +195,20 @@ def tune(mech: str, cells: int = 32 ** 3, seed: int = 7, reps: int = 2, log=prin
                     torch.cuda.synchronize()
                     errs.append(float((Sb[..., :nsp] - ref[..., :nsp]).abs().max()))
                     terrs.append(float((Sb[..., nsp] - ref[..., nsp]).abs().max()))
-                name = "%s n%d r%.1f a%d guess%d h0f%g tol%.1e" % ("adaptive" if adaptive else "fixed", nsub,
-                                                                 grade, att, guess, h0f, tol)
+                name = "%s n%d r%.1f a%d guess%d h0f%g jac%d tol%.1e" % (
+                    "adaptive" if adaptive else "fixed", nsub, grade, att, guess, h0f, mj, tol)
                 st_all[name] = {"ms_generated_plus_relaxed": t_tot * 1e3, "max_species_err_generated": errs[0],
                                 "max_species_err_relaxed": errs[1], "max_T_err_K": max(terrs)}
                 log("  surrogate %s: %.2f ms, max |dY| %.2e / %.2e, max |dT| %.2e K" % (
                     name, t_tot * 1e3, errs[0], errs[1], max(terrs)))
                 # half the 1e-6 bound: larger blocks reach further into the tail
                 if max(errs) <= MAX_SPECIES_ERR and max(terrs) <= MAX_T_ERR and (best is None or t_tot < best[0]):
-                    best = (t_tot, adaptive, nsub, grade, att, guess, h0f, tol)
+                    best = (t_tot, adaptive, nsub, grade, att, guess, h0f, mj, tol)
         rec["measured"]["surrogate"] = st_all
         if best:
             (_, rec["surr_adaptive"], rec["surr_nsub"], rec["surr_grade"], rec["surr_max_att"], rec["surr_guess"],
-             rec["surr_h0_fac"], rec["surr_tol_y"]) = best
+             rec["surr_h0_fac"], rec["surr_max_jac"], rec["surr_tol_y"]) = best
+            rec["surr_jac_theta"] = SURR_JAC_THETA
             rec["measured"]["surrogate_speedup"] = (sets[0][5] + sets[1][5]) / best[0]
         fc.set_surrogate(None, None, None, None)
     fc.close()
@@ -223,8 +245,8 @@ def apply(fc, cells: int, path=CACHE) -> dict:
     done = {}
     if not rec:
         return done
-    for k in ("rhs_mapping", "grid", "surr_adaptive", "surr_nsub", "surr_grade", "surr_max_att", "surr_guess",
-              "surr_h0_fac", "surr_tol_y"):
+    for k in ("rhs_mapping", "grid", "jacobian", "surr_adaptive", "surr_nsub", "surr_grade", "surr_max_att",
+              "surr_guess", "surr_h0_fac", "surr_jac_theta", "surr_max_jac", "surr_tol_y"):
         if k in rec:
             fc.set_option(k, rec[k])
             done[k] = rec[k]

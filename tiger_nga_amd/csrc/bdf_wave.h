@@ -833,8 +833,19 @@ struct Bdf {
   static constexpr double kSdBh2 = (0.5 - kSdG) / (kSdC2 - kSdG);
   static constexpr double kSdBh1 = 1.0 - kSdBh2;
 
+  // Simplified Newton for one stage value z = base + gh f(z), the iteration
+  // matrix's LU factors in M.  The iterate's remaining error is about
+  // theta / (1 - theta) times the last correction, theta the contraction rate
+  // |d_k| / |d_k-1|, so the iteration stops when max(1, theta / (1 - theta))
+  // |d| <= ntol (in acceptance tolerances): never looser than |d| <= ntol, and
+  // tighter where a stale Jacobian makes the iteration contract slowly.  theta
+  // enters as the previous solve's rate (the first correction has none of its
+  // own) and leaves as this solve's last; thmax collects the largest rate
+  // measured.  Fails on a non-finite or non-contracting (theta >= 0.99)
+  // correction or after nmax iterations.
   __device__ __forceinline__ bool sdirk_newton(double& z, double base, double gh, double wacc, int nmax, double ntol,
-                                               int& nit) {
+                                               int& nit, double& theta, double& thmax) {
+    double dprev = 0.0;
     for (int it = 0; it < nmax; ++it) {
       const double fz = f(z);
       c_nfe++;
@@ -843,8 +854,15 @@ struct Bdf {
       nit++;
       if (act) z += d;
       const double dn = wave_max(act ? fabs(d) * wacc : 0.0);
-      if (dn <= ntol) return true;
       if (!isfinite(dn)) return false;
+      if (it > 0) {
+        theta = dn / dprev;
+        thmax = fmax(thmax, theta);
+        if (theta >= 0.99) return false;
+      }
+      const double th = fmin(theta, 0.9);
+      if (fmax(1.0, th / (1.0 - th)) * dn <= ntol) return true;
+      dprev = dn;
     }
     return false;
   }
@@ -865,10 +883,15 @@ struct Bdf {
   // Returns 0 (accepted), or after max_att attempted steps without reaching
   // dt (or a non-finite error estimate) 1 if the last failed attempt was a
   // Newton or singular-matrix failure, 2 if it failed the error test.
+  //
+  // Jacobian updates: when a step attempt's Newton iterations contracted
+  // slower than jac_theta (or failed) with a Jacobian taken at an earlier
+  // state, the next attempt first evaluates the Jacobian at the current state
+  // and refactors, up to max_jac Jacobians per cell (1: the initial one only).
   template <class GuessAt>
   __device__ __forceinline__ int sdirk_adaptive(double& yv, double dt, double h0, double h0_fac, const GuessAt& gat,
                                                 bool use_guess, double wacc, int nmax, double ntol, int max_att,
-                                                double grow, double keep, int& nit) {
+                                                double grow, double keep, double jac_theta, int max_jac, int& nit) {
     c_nfe = c_nfe_dq = c_nje = c_nsetups = c_nni = c_netf = c_ncfn = c_nlu = 0;
     nst = 0;
     const double y0 = act ? yv : 0.0;
@@ -901,11 +924,31 @@ struct Bdf {
     double kl = act ? f0 : 0.0;  // derivative at yi (the last stage's, the method being stiffly accurate)
     int why = 2;  // reason of the last failed attempt: 1 Newton / singular, 2 error test
     double h_lu = -1.0;  // step size of the factors held in M (-1: none)
+    double theta = 0.0;  // Newton contraction rate, carried from solve to solve
+    bool jfresh = true, refresh = false;  // Jacobian taken at yi; update it before the next attempt
     for (int att = 0; att < max_att; ++att) {
       if (dt - t <= 1.0e-12 * dt) {
         yv = yi;
         return 0;
       }
+      if (refresh && c_nje < max_jac) {
+        if constexpr (AJ) {
+          jac_analytic(yi);
+        } else if (rc) {
+          const double fi = jac_split_base(yi);
+          jac_split_cols(yi, fi);
+        } else {
+          const double fi = f(yi);
+          c_nfe++;
+          dq_jac(yi, fi);
+        }
+        c_nje++;
+        jfresh = true;
+        h_lu = -1.0;
+        theta = 0.0;
+      }
+      refresh = false;
+      double thmax = 0.0;
       if (t + hh > dt || dt - (t + hh) < 1.0e-3 * hh) hh = dt - t;
       const double gh = kSdG * hh;
       if (hh != h_lu) {
@@ -937,9 +980,10 @@ struct Bdf {
         z = yi + (gc - gp);
         gp = gc;
       }
-      if (!sdirk_newton(z, yi, gh, wacc, nmax, ntol, nit)) {
+      if (!sdirk_newton(z, yi, gh, wacc, nmax, ntol, nit, theta, thmax)) {
         hh *= 0.25;  // no convergence: retry with a smaller step
         why = 1;
+        refresh = !jfresh;
         continue;
       }
       const double k1 = act ? (z - yi) * rg : 0.0;
@@ -951,9 +995,10 @@ struct Bdf {
       } else {
         z = act ? base2 + gh * k1 : 0.0;
       }
-      if (!sdirk_newton(z, base2, gh, wacc, nmax, ntol, nit)) {
+      if (!sdirk_newton(z, base2, gh, wacc, nmax, ntol, nit, theta, thmax)) {
         hh *= 0.25;
         why = 1;
+        refresh = !jfresh;
         continue;
       }
       const double k2 = act ? (z - base2) * rg : 0.0;
@@ -964,9 +1009,10 @@ struct Bdf {
       } else {
         z = act ? base3 + gh * k2 : 0.0;
       }
-      if (!sdirk_newton(z, base3, gh, wacc, nmax, ntol, nit)) {
+      if (!sdirk_newton(z, base3, gh, wacc, nmax, ntol, nit, theta, thmax)) {
         hh *= 0.25;
         why = 1;
+        refresh = !jfresh;
         continue;
       }
       const double k3 = act ? (z - base3) * rg : 0.0;
@@ -980,10 +1026,12 @@ struct Bdf {
         t += hh;
         yi = z;
         kl = k3;
+        jfresh = false;
         if (fac >= 1.0 && fac < keep) fac = 1.0;  // keep h and reuse the factors
       } else {
         why = 2;
       }
+      refresh = thmax > jac_theta && !jfresh;
       hh *= fac;
     }
     if (dt - t <= 1.0e-12 * dt) {
@@ -1012,6 +1060,7 @@ struct Bdf {
     }
     c_nje++;
     double yi = y0;
+    double theta = 0.0, thmax = 0.0;  // Newton contraction rate (carried), largest seen
     // Newton starts from the previous stage's converged value plus the
     // network's predicted change between the two stage times (its bias,
     // smooth in time, cancels); gp: the network's value at the previous stage
@@ -1023,10 +1072,11 @@ struct Bdf {
       gp = gc;
       return z0;
     };
+    double h_lu = -1.0;  // step size of the factors held in M (equal sub-steps share them)
     for (int i = 0; i < nsub; ++i) {
       const double hh = hs[i];
       const double gh = kSdG * hh;
-      {
+      if (hh != h_lu) {
         const TGR_GLOBAL double* jr = fresh(jw) + lane;
 #pragma unroll
         for (int c = 0; c < NEQ; ++c) {
@@ -1034,20 +1084,21 @@ struct Bdf {
           if (c == lane) v += 1.0;
           M[c] = v;
         }
+        c_nlu++;
+        if (getrf()) return 1;
+        h_lu = hh;
       }
-      c_nlu++;
-      if (getrf()) return 1;
       const double rg = 1.0 / gh;
       double z = start(i, 0, yi);
-      if (!sdirk_newton(z, yi, gh, wacc, nmax, ntol, nit)) return 1;
+      if (!sdirk_newton(z, yi, gh, wacc, nmax, ntol, nit, theta, thmax)) return 1;
       const double k1 = act ? (z - yi) * rg : 0.0;
       const double base2 = act ? yi + kSdA21 * hh * k1 : 0.0;
       z = start(i, 1, z);
-      if (!sdirk_newton(z, base2, gh, wacc, nmax, ntol, nit)) return 1;
+      if (!sdirk_newton(z, base2, gh, wacc, nmax, ntol, nit, theta, thmax)) return 1;
       const double k2 = act ? (z - base2) * rg : 0.0;
       const double base3 = act ? yi + hh * (kSdB1 * k1 + kSdB2 * k2) : 0.0;
       z = start(i, 2, z);
-      if (!sdirk_newton(z, base3, gh, wacc, nmax, ntol, nit)) return 1;
+      if (!sdirk_newton(z, base3, gh, wacc, nmax, ntol, nit, theta, thmax)) return 1;
       const double k3 = act ? (z - base3) * rg : 0.0;
       const double e =
           getrs(act ? hh * ((kSdB1 - kSdBh1) * k1 + (kSdB2 - kSdBh2) * k2 + kSdG * k3) : 0.0);

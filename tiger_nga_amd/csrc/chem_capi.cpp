@@ -107,7 +107,10 @@ struct tgr_chem {
   // every surr_probe calls (0: always guided)
   int surr_probe = 8, surr_skipped = 0;
   int surr_adaptive = 1, surr_max_att = 48, surr_split_jac = 1;
-  double surr_h0_fac = 0.0, surr_grow = 10.0, surr_keep = 1.5;
+  int surr_mode = 0;        // 0: guided solve; 1: standalone surrogate with an FP64 residual check
+  double surr_sa_tol = 1.0;  // standalone: largest Newton correction accepted, in acceptance tolerances
+  double surr_h0_fac = 0.0, surr_grow = 10.0, surr_keep = 1.5, surr_jac_theta = 0.3;
+  int surr_max_jac = 4;
   double surr_min_accept = 0.15, surr_last_accept = 1.0;
   double surr_newton_tol = 0.1, surr_tol_y = 2.5e-7, surr_tol_t = 1.0e-2, surr_grade = 2.0;
   hipEvent_t ev2 = nullptr;
@@ -427,6 +430,12 @@ int tgr_chem_set_option(tgr_chem* c, const char* key, int64_t v) {
     c->surr_probe = (int)v;
     c->surr_skipped = 0;
     c->surr_last_accept = 1.0;
+  } else if (!strcmp(key, "surr_max_jac")) {
+    if (v < 1 || v > 64) return fail(TGR_E_ARG, "surr_max_jac must be 1..64");
+    c->surr_max_jac = (int)v;
+  } else if (!strcmp(key, "surr_mode")) {
+    if (v != 0 && v != 1) return fail(TGR_E_ARG, "surr_mode must be 0 (guided solve) or 1 (standalone surrogate)");
+    c->surr_mode = (int)v;
   } else if (!strcmp(key, "surr_newton_max")) {
     if (v < 1 || v > 50) return fail(TGR_E_ARG, "surr_newton_max must be 1..50");
     c->surr_newton_max = (int)v;
@@ -513,10 +522,16 @@ int tgr_chem_set_option_real(tgr_chem* c, const char* key, double v) {
   } else if (!strcmp(key, "surr_keep")) {
     if (v < 1.0 || v > 4.0) return fail(TGR_E_ARG, "surr_keep must be in [1, 4]");
     c->surr_keep = v;
+  } else if (!strcmp(key, "surr_jac_theta")) {
+    if (!(v > 0.0) || v >= 1.0) return fail(TGR_E_ARG, "surr_jac_theta must be in (0, 1)");
+    c->surr_jac_theta = v;
   } else if (!strcmp(key, "surr_tol_y")) c->surr_tol_y = v;
   else if (!strcmp(key, "surr_tol_t")) c->surr_tol_t = v;
   else if (!strcmp(key, "surr_newton_tol")) c->surr_newton_tol = v;
-  else if (!strcmp(key, "surr_min_accept")) {
+  else if (!strcmp(key, "surr_sa_tol")) {
+    if (!(v > 0.0)) return fail(TGR_E_ARG, "surr_sa_tol must be positive");
+    c->surr_sa_tol = v;
+  } else if (!strcmp(key, "surr_min_accept")) {
     if (v > 1.0) return fail(TGR_E_ARG, "surr_min_accept must be in (0, 1]");
     c->surr_min_accept = v;
     c->surr_skipped = 0;
@@ -575,10 +590,16 @@ int run_surrogate(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, 
   }
   c->surr_skipped = 0;
   if (sstats) sstats->guided = 1;
+  // standalone surrogate (surr_mode 1): the fixed schedule with one FP64
+  // Newton correction per stage from the network's stage values, the stage
+  // accepted when that correction -- the distance of the prediction from the
+  // stage equation's solution -- is within surr_sa_tol acceptance tolerances
+  const bool standalone = c->surr_mode == 1;
+  const int adaptive = standalone ? 0 : adaptive;
   const int64_t rows = (p.ncell + 63) / 64 * 64;
   // network output times: every stage of the fixed schedule, or the end of
   // each sub-step of the schedule for the adaptive solve (which interpolates)
-  const int nst = c->surr_adaptive ? c->surr_nsub : tgr::kSurrStages * c->surr_nsub;
+  const int nst = adaptive ? c->surr_nsub : tgr::kSurrStages * c->surr_nsub;
   // sub-step sizes: geometric with ratio surr_grade (1: equal), summing to dt
   double hsub[tgr::kSurrMaxSub] = {0.0};
   {
@@ -591,7 +612,7 @@ int run_surrogate(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, 
     }
     hsub[n - 1] += dt - acc;  // exact sum
   }
-  const int grid_s = persistent_grid(c, tgr_surr_occupancy(c->neq, c->surr_adaptive), p.ncell, tgr_surr_block_waves());
+  const int grid_s = persistent_grid(c, tgr_surr_occupancy(c->neq, adaptive), p.ncell, tgr_surr_block_waves());
   const int grid_p = persistent_grid(c, tgr_step_occupancy(c->neq), p.ncell, tgr_step_block_waves());
   // the workspace serves both kernels: each wave of each block owns neq x 64 doubles
   const size_t ws_waves = std::max((size_t)grid_s * tgr_surr_block_waves(), (size_t)grid_p * tgr_step_block_waves());
@@ -613,9 +634,9 @@ int run_surrogate(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, 
     static const double cst[tgr::kSurrStages] = {0.43586652150845899942, 0.5 * (1.0 + 0.43586652150845899942), 1.0};
     double t0 = 0.0;
     for (int k = 0; k < nst; ++k) {
-      const int i = c->surr_adaptive ? k : k / tgr::kSurrStages;
-      const int j = c->surr_adaptive ? tgr::kSurrStages - 1 : k % tgr::kSurrStages;
-      if ((c->surr_adaptive || j == 0) && i > 0) t0 += hsub[i - 1];
+      const int i = adaptive ? k : k / tgr::kSurrStages;
+      const int j = adaptive ? tgr::kSurrStages - 1 : k % tgr::kSurrStages;
+      if ((adaptive || j == 0) && i > 0) t0 += hsub[i - 1];
       const double tau = t0 + cst[j] * hsub[i];
       const float v = (float)((std::log10(tau) - c->h_in_shift[col]) * c->h_in_scale[col]);
       if (tgr_launch_surr_set_column(X, rows, c->net_in_pad, col, v, s)) return fail(TGR_E_HIP, "column launch");
@@ -627,21 +648,23 @@ int run_surrogate(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, 
   P.s = p;
   P.s.queue = c->d_queue;
   P.s.ws = (double*)c->b_ws.p;
-  P.adaptive = c->surr_adaptive;
+  P.adaptive = adaptive;
   P.max_att = c->surr_max_att;
   P.h0_fac = c->surr_h0_fac;
   P.ntau = nst;
   P.grow = c->surr_grow;
   P.split_jac = c->surr_split_jac;
   P.keep = c->surr_keep;
+  P.jac_theta = c->surr_jac_theta;
+  P.max_jac = c->surr_max_jac;
   {
     // times of the network outputs, in the order they were computed
     static const double cs[tgr::kSurrStages] = {0.43586652150845899942, 0.5 * (1.0 + 0.43586652150845899942), 1.0};
     double t0 = 0.0;
     for (int k = 0; k < nst; ++k) {
-      const int i = c->surr_adaptive ? k : k / tgr::kSurrStages;
-      const int j = c->surr_adaptive ? tgr::kSurrStages - 1 : k % tgr::kSurrStages;
-      if ((c->surr_adaptive || j == 0) && i > 0) t0 += hsub[i - 1];
+      const int i = adaptive ? k : k / tgr::kSurrStages;
+      const int j = adaptive ? tgr::kSurrStages - 1 : k % tgr::kSurrStages;
+      if ((adaptive || j == 0) && i > 0) t0 += hsub[i - 1];
       P.tau[k] = t0 + cs[j] * hsub[i];
     }
   }
@@ -653,8 +676,8 @@ int run_surrogate(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, 
   P.elem_w = c->d_elem;
   P.nsub = c->surr_nsub;
   P.use_guess = c->surr_guess;
-  P.newton_max = c->surr_newton_max;
-  P.newton_tol = c->surr_newton_tol;
+  P.newton_max = standalone ? 1 : c->surr_newton_max;
+  P.newton_tol = standalone ? c->surr_sa_tol : c->surr_newton_tol;
   P.tol_y = c->surr_tol_y;
   P.tol_t = c->surr_tol_t;
   P.rej_count = c->d_rej_count;

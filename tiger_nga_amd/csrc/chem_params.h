@@ -69,6 +69,8 @@ struct SurrParams {
   int split_jac;                 // 1: Jacobian with the rate constants evaluated once (Bdf::jac_split)
   double grow;                   // adaptive: largest step growth factor
   double keep;                   // adaptive: growth factors in [1, keep) keep h (and the LU factors)
+  double jac_theta;              // adaptive: Newton contraction rate above which the Jacobian is re-evaluated
+  int max_jac;                   // adaptive: Jacobians per cell (1: the initial one only)
   double tau[kSurrStages * kSurrMaxSub];  // times of the network outputs, increasing
   int newton_max;
   double newton_tol;             // Newton stop: |correction| <= newton_tol x acceptance tolerance

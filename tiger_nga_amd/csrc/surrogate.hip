@@ -133,7 +133,7 @@ __device__ __forceinline__ int surr_cell(const SurrParams& P, const DevMech& m, 
       return sol0 + rho * osc * (v0 + w * (v1 - v0));
     };
     st = b.sdirk_adaptive(sol, p.dt, P.hsub[0], P.h0_fac, gat, NET, wacc, P.newton_max, P.newton_tol, P.max_att,
-                          P.grow, P.keep, nit);
+                          P.grow, P.keep, P.jac_theta, P.max_jac, nit);
   } else {
     st = b.sdirk_guided(sol, P.nsub, P.hsub, guess, NET, wacc, P.newton_max, P.newton_tol, nit);
   }

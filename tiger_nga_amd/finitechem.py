@@ -40,7 +40,7 @@ class _Stats(ctypes.Structure):
 
 
 REAL_OPTIONS = ("surr_tol_y", "surr_tol_t", "surr_newton_tol", "surr_grade", "surr_min_accept", "surr_h0_fac",
-                "surr_grow", "surr_keep")
+                "surr_grow", "surr_keep", "surr_jac_theta", "surr_sa_tol")
 
 
 class _SurrStats(ctypes.Structure):
@@ -222,10 +222,9 @@ class FiniteChem:
             pass
 
     def set_option(self, key: str, value):
-        """Integer knobs ("grid", "rhs_mapping", "max_order", "max_steps",
-        "surr_nsub", "surr_guess", "surr_newton_max", "surr_probe", "surr_adaptive", "surr_max_att") or real ones
-        ("surr_tol_y", "surr_tol_t", "surr_newton_tol", "surr_grade", "surr_min_accept", "surr_h0_fac",
-        "surr_grow", "surr_keep")."""
+        """Integer knobs ("grid", "rhs_mapping", "jacobian", "max_order", "max_steps", "surr_nsub",
+        "surr_guess", "surr_newton_max", "surr_probe", "surr_adaptive", "surr_max_att", "surr_max_jac") or real
+        ones (REAL_OPTIONS)."""
         if key in REAL_OPTIONS:
             _check(self.lib, self.lib.tgr_chem_set_option_real(self._h, key.encode(), float(value)), "set_option")
         else:
