@@ -34,6 +34,10 @@
  *   orc_jacobian    analytic Jacobian of finitechem_compute_rhs (no reference
  *                   routine; CVODE differences the RHS, finitechem.f90:1028),
  *                   pinned by central differences of orc_rhs.
+ *   orc_diffusion   finitechem_diffusion (finitechem.f90:538-923), the
+ *                   molar-diffusion correction and enthalpy flux; pinned by
+ *                   flux conservation, closed forms on linear fields and
+ *                   planted mistakes (tests/test_diffusion.py).
  */
 #pragma once
 #include <stdint.h>
@@ -108,6 +112,31 @@ typedef struct {
 int orc_source_chemistry(int mech, const orc_opts *o, double dt, double p_thermo, double p_thermo_old,
                          const orc_grid *g, double *RHO_, double *SC_, const int *mask,
                          double *HR_gp, double *chemsrc_gp, orc_stats *total, int nthreads);
+
+/* Metric operators of metric_generic (metric/metric_generic.f90:46-84), Fortran
+ * arrays with the reference's extents: interp_sc_x(imin_-st1:imax_+st2,
+ * jmino_:jmaxo_, -st2:st1); interp_sc_y, interp_sc_z(imino_:imaxo_,
+ * jmin_-st1:jmax_+st2, -st2:st1); grad_x/y/z(imin_-st1:imax_+st2,
+ * jmin_-st1:jmax_+st2, -st2:st1); div_u/v/w and interp_u_xm/v_ym/w_zm
+ * (imin_:imax_, jmin_:jmax_, -st1:st2); grad_xm/ym/zm(imin_:imax_, jmin_:jmax_,
+ * -stp:stp), stp = st1 + st2. */
+typedef struct {
+  int st1, st2;
+  const double *interp_sc_x, *interp_sc_y, *interp_sc_z;
+  const double *grad_x, *grad_y, *grad_z;
+  const double *div_u, *div_v, *div_w;
+  const double *interp_u_xm, *interp_v_ym, *interp_w_zm;
+  const double *grad_xm, *grad_ym, *grad_zm;
+} orc_metric;
+
+/* finitechem_diffusion (finitechem.f90:538-923): diff_corr(..., nsp+1) and
+ * diff_gp(..., 3) over the ghosted block (interior cells written, the rest 0),
+ * from SC_, DIFF_ (both (..., nscalar)), RHO_ and mask; molar / gasrad / cv:
+ * the 'Molar diffusion' and 'Gas radiation' parser keys and compressible or
+ * constant volume. */
+int orc_diffusion(int mech, const orc_grid *g, const orc_metric *m, double dt, int molar, int gasrad, int cv,
+                  const double *RHO_, const double *SC_, const double *DIFF_, const int *mask, double *diff_corr,
+                  double *diff_gp);
 
 #ifdef __cplusplus
 }

@@ -177,3 +177,43 @@ def source_chemistry(mech, RHO_, SC_, g, dt, p_thermo, p_thermo_old=None, mask=N
     lib().orc_source_chemistry(MECH_ID[mech], ctypes.byref(o), dt, p_thermo, p_thermo_old, ctypes.byref(cg),
                                _p(RHO_), _p(SC_), ptr(mask), ptr(HR_gp), ptr(chemsrc_gp), ctypes.byref(st), nthreads)
     return {n: getattr(st, n) for n, _ in Stats._fields_}
+
+
+class Metric(ctypes.Structure):
+    _fields_ = [("st1", ctypes.c_int), ("st2", ctypes.c_int)] + [(n, ctypes.c_void_p) for n in (
+        "interp_sc_x", "interp_sc_y", "interp_sc_z", "grad_x", "grad_y", "grad_z", "div_u", "div_v", "div_w",
+        "interp_u_xm", "interp_v_ym", "interp_w_zm", "grad_xm", "grad_ym", "grad_zm")]
+
+
+def diffusion(mech, g, metric, dt, RHO_, SC_, DIFF_, mask=None, molar=True, gasrad=False, cv=False):
+    """finitechem_diffusion on Fortran-ordered numpy fields; metric: dict of
+    Fortran-ordered operator arrays plus st1, st2.  Returns (diff_corr, diff_gp)."""
+    L = lib()
+    L.orc_diffusion.argtypes = [ctypes.c_int, ctypes.POINTER(Grid), ctypes.POINTER(Metric), ctypes.c_double,
+                                ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_void_p, _dp, _dp]
+    cg = Grid(g.imin_, g.imax_, g.jmin_, g.jmax_, g.kmin_, g.kmax_, g.imino_, g.imaxo_, g.jmino_, g.jmaxo_,
+              g.kmino_, g.kmaxo_, g.imino_m, g.imaxo_m, g.jmino_m, g.jmaxo_m, g.nscalar, g.isc_sc, g.isc_T)
+    keep = {k: np.asfortranarray(v, dtype=np.float64) for k, v in metric.items() if k not in ("st1", "st2")}
+    m = Metric(metric["st1"], metric["st2"], *[keep[n].ctypes.data for n, _ in Metric._fields_[2:]])
+    nsp = orc_info_nsp(mech)
+    sh = RHO_.shape
+    dc = np.zeros(sh + (nsp + 1,), order="F")
+    gp = np.zeros(sh + (3,), order="F")
+    mk = None if mask is None else np.asfortranarray(mask, dtype=np.int32)
+    rc = L.orc_diffusion(MECH_ID[mech], ctypes.byref(cg), ctypes.byref(m), dt, int(molar), int(gasrad), int(cv),
+                         _p(np.asfortranarray(RHO_)), _p(np.asfortranarray(SC_)), _p(np.asfortranarray(DIFF_)),
+                         None if mk is None else mk.ctypes.data, _p(dc), _p(gp))
+    assert rc == 0, rc
+    return dc, gp
+
+
+class _Info(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("nsp", "nrxn", "nslot", "n_third")]
+
+
+def orc_info_nsp(mech):
+    L = lib()
+    L.orc_mech_info.argtypes = [ctypes.c_int, ctypes.POINTER(_Info)]
+    inf = _Info()
+    L.orc_mech_info(MECH_ID[mech], ctypes.byref(inf))
+    return inf.nsp
