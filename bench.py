@@ -217,9 +217,12 @@ def bench_surrogate(a, torch, fc, g, R_d, S0_d, S_d, HR_d, CS_d, dt, P, mech, sp
     _progress("guided step (default policy), %d + %d steps" % (a.warmup, a.steps))
     t_head = _time_steps(torch, guided, S_d, S0_d, a.steps, a.warmup)
     S_d.copy_(S0_d)
-    _, ss = fc.source_chemistry_surrogate(R_d, S_d, g, dt, P, P, HR_gp=HR_d, chemsrc_gp=CS_d, stream=sp)
-    state = S_d.clone()
+    _, sh = fc.source_chemistry_surrogate(R_d, S_d, g, dt, P, P, HR_gp=HR_d, chemsrc_gp=CS_d, stream=sp)
+    state = S_d.clone()  # what the product call under its policy returns
+    # the guided solve's own counters and time: one call with the solve forced
     fc.set_option("surr_probe", 0)
+    S_d.copy_(S0_d)
+    _, ss = fc.source_chemistry_surrogate(R_d, S_d, g, dt, P, P, HR_gp=HR_d, chemsrc_gp=CS_d, stream=sp)
     _progress("guided step forced on every call, %d steps" % side)
     t_forced = _time_steps(torch, guided, S_d, S0_d, side, 1)
     # the other network setting on the same states: what the network's
@@ -268,6 +271,7 @@ def bench_surrogate(a, torch, fc, g, R_d, S0_d, S_d, HR_d, CS_d, dt, P, mech, sp
         "value": cells / t_head, "ms_per_step": t_head * 1e3, "model": os.path.relpath(path, ROOT),
         "policy": "adaptive use: guided solve while its accepted fraction >= 0.15, else plain step with a guided "
                   "probe every 8 calls",
+        "policy_last_call_guided": bool(sh["guided"]),
         "forced_guided": {"value": cells / t_forced, "ms_per_step": t_forced * 1e3, "steps": side,
                           "accepted_fraction": ss["accepted"] / max(1, cells),
                           "rejected": {"newton": ss["rej_newton"], "error_estimate": ss["rej_error"],

@@ -139,6 +139,49 @@ int tgr_chem_properties_device(tgr_chem *ctx, double p_thermo, const tgr_grid *g
                                const int *mask, const double *Lewis, int isc_zmix, double *RHO_, double *VISC_,
                                double *DIFF_, void *stream);
 
+/* Metric operators of the flow solver (metric_generic, reference
+ * metric/metric_generic.f90:46-84), DEVICE arrays in Fortran layout with the
+ * reference's extents (st1 >= 0, st2 >= 1: stencil widths, order = 2 st2 and
+ * st1 = st2 - 1 in the reference; stp = st1 + st2):
+ *   interp_sc_x(imin_-st1:imax_+st2, jmino_:jmaxo_, -st2:st1)
+ *   interp_sc_y, interp_sc_z(imino_:imaxo_, jmin_-st1:jmax_+st2, -st2:st1)
+ *   grad_x, grad_y, grad_z(imin_-st1:imax_+st2, jmin_-st1:jmax_+st2, -st2:st1)
+ *   div_u/v/w, interp_u_xm / v_ym / w_zm(imin_:imax_, jmin_:jmax_, -st1:st2)
+ *   grad_xm / ym / zm(imin_:imax_, jmin_:jmax_, -stp:stp) */
+typedef struct {
+  int st1, st2;
+  const double *interp_sc_x, *interp_sc_y, *interp_sc_z;
+  const double *grad_x, *grad_y, *grad_z;
+  const double *div_u, *div_v, *div_w;
+  const double *interp_u_xm, *interp_v_ym, *interp_w_zm;
+  const double *grad_xm, *grad_ym, *grad_zm;
+} tgr_metric;
+
+#define TGR_DIFF_MOLAR 1   /* 'Molar diffusion' (finitechem.f90:254, default on) */
+#define TGR_DIFF_GASRAD 2  /* 'Gas radiation' (finitechem.f90:216) */
+#define TGR_DIFF_CV 4      /* compressible or constant volume: cv instead of cp */
+
+/* finitechem_diffusion (finitechem.f90:538-923) on DEVICE arrays, and with
+ * src != NULL the update of finitechem_source_transport (492-528): for every
+ * interior cell with mask /= 1 writes diff_corr(..., 1:nsp) = dt div F_k (the
+ * species' corrected molar-diffusion flux), diff_corr(..., nsp+1) = dt (sum_k
+ * cp_k interp(F_k total) grad T [+ gas radiation] + DIFF_T grad cp grad T) /
+ * cp, and diff_gp(..., 1:3) = div(Fickian flux) + the correction of H2O, CO,
+ * CO2 (0 where the mechanism lacks the species is left untouched), and adds
+ * diff_corr to src's species block and T (same layout as SC_).  RHO_(...),
+ * SC_ and DIFF_(..., nscalar) as in the step (DIFF_ holds rho D per species and
+ * lambda / cp at isc_T); mask(imino:imaxo, jmino:jmaxo) must cover the
+ * array extents; the block needs st1 + st2 ghost cells on every side.  Any of
+ * diff_corr, diff_gp, src may be NULL; cells outside the interior (and wall
+ * cells) are not written.  Readings of the reference's non-conforming lines:
+ * SC(i-st2:j+st1) at 652 as i+st1; z sections k-st2:k-st1 at 634-635 and 706
+ * as k-st2:k+st1 (identical at second order); Cp_sp(isc) at 720-730 as the
+ * species' own cp.  The burner heat loss (796-802) and the external ignition
+ * source (842-859), fixed to single cases' meshes, stay with the caller. */
+int tgr_chem_diffusion_device(tgr_chem *ctx, double dt, const tgr_grid *g, const tgr_metric *m, int flags,
+                              const double *RHO_, const double *SC_, const double *DIFF_, const int *mask,
+                              double *diff_corr, double *diff_gp, double *src, void *stream);
+
 /* Temperature from mixture enthalpy for n states, device arrays, SoA:
  * Y[nsp][n] mass fractions, hmix[n] J/kg, T0[n] K starting guesses (NULL or
  * <= 0: 1500 K) -> T[n] K, iters[n] (may be NULL): Newton iterations, -1 when
@@ -257,6 +300,12 @@ int tgr_chem_set_option_real(tgr_chem *ctx, const char *key, double value);
  *                 from a device work counter
  *   "rhs_mapping" tgr_chem_rhs_device: 1 (default) one state per lane (generated
  *                 straight-line code of the mechanism), 0 one state per wavefront
+ *   "step_mapping"  0 (default): a wavefront per cell (the species in lanes);
+ *                 1: a thread per cell -- the same CVODE-semantics BDF with the
+ *                 cell's vectors and Nordsieck history in the thread's registers
+ *                 and its iteration matrix in LDS -- for the mechanisms small
+ *                 enough for it (H2, neq 10; others, and "jacobian" 1, keep the
+ *                 wave per cell); the autotuner times both
  *   "jacobian"    0 (default): the Jacobian by difference quotients, one
  *                 right-hand side per column, as CVODE's FCVDENSE builds it for
  *                 the reference (finitechem.f90:1028); 1: the analytic Jacobian

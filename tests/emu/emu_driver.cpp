@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "bdf_lane.h"
 #include "bdf_wave.h"
 #include "lane_rhs_dme39.h"
 #include "lane_rhs_ethyred.h"
@@ -277,6 +278,35 @@ extern "C" int emu_lane_rhs(const char* mech, long n, double p_gp, const double*
     case 40: run_n(std::integral_constant<int, 40>()); break;
     case 46: run_n(std::integral_constant<int, 46>()); break;
     default: return -3;
+  }
+  return 0;
+}
+
+// The thread-per-cell solver (LaneBdf, bdf_lane.h) on ncell packed cells, one
+// cell after another on this host thread (it has no cross-lane operations):
+// sol[ncell][neq] integrated in place, counters[ncell * 10] as emu_step.
+extern "C" int emu_lane_step(const char* mech, long ncell, double* sol, double dt, double p_gp, double rtol,
+                             double atol, int max_steps, int* counters) {
+  const tgr_mech_tables* t = find_mech(mech);
+  if (!t || t->nsp + 1 != 10) return -1;
+  constexpr int NEQ = 10;
+  std::vector<double> ws((size_t)NEQ * NEQ * tgr::kWave), ml((size_t)NEQ * NEQ * tgr::kWave);
+  for (long c = 0; c < ncell; ++c) {
+    tgr::LaneBdf<NEQ> b;
+    b.p_gp = p_gp;
+    b.rtol = rtol;
+    b.atol = atol;
+    b.qmax = tgr::kQmaxAbs;
+    b.max_steps = max_steps;
+    b.jw = ws.data();
+    b.Ml = ml.data();
+    double s[NEQ];
+    for (int k = 0; k < NEQ; ++k) s[k] = sol[c * NEQ + k];
+    const int st = b.integrate(dt, s);
+    for (int k = 0; k < NEQ; ++k) sol[c * NEQ + k] = s[k];
+    const int cnt[TGR_NCOUNT] = {b.nst, b.c_nfe, b.c_nfe_dq, b.c_nje, b.c_nsetups, b.c_nni, b.c_netf, b.c_ncfn,
+                                 b.c_nlu, st};
+    for (int q = 0; q < TGR_NCOUNT; ++q) counters[c * TGR_NCOUNT + q] = cnt[q];
   }
   return 0;
 }

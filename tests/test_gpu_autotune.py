@@ -1,6 +1,7 @@
 """The autotuner measures its candidates on the GPU, records a valid choice
-for each knob, and applying the choices leaves the step's results unchanged
-(the grid and the batched right-hand-side mapping change scheduling only)."""
+for each knob, and applying the choices gives the step's results bit for bit
+as the same Jacobian and step mapping set by hand (the grid and the batched
+right-hand-side mapping change scheduling only)."""
 import pytest
 
 pytestmark = pytest.mark.gpu
@@ -26,6 +27,11 @@ def test_tune_and_apply_h2(tmp_path):
     autotune.save(rec, 16 ** 3, cache)
     fc = FiniteChem("h2")
     R_d, S0, g, P = surrogate.block_states(fc, "h2", 16 ** 3, 5, 0)
+    # the plain step's Jacobian and mapping change the rounding; the grid and
+    # the batched right-hand side's mapping do not
+    for k in ("jacobian", "step_mapping"):
+        if k in rec:
+            fc.set_option(k, rec[k])
     ref = S0.clone()
     fc.source_chemistry(R_d, ref, g, 1e-6, P, P)
     done = autotune.apply(fc, 16 ** 3, cache)

@@ -250,3 +250,55 @@ def test_step_failure_paths(chem, oracle, torch_dev):
     for c in (0, 3):
         assert abs(S[c, 0, 0, isc_T - 1] - ref[c, -1] / rho[c]) < 1e-6
     print("\nfailure paths: MAX_NSTEPS=5 -> 8/8 failed with the oracle's states; NaN / negative T -> 2 failed")
+
+
+@pytest.mark.parametrize("rtol,atol", [(1e-6, 1e-12), (1e-10, 1e-16)])
+def test_lane_mapping_step_vs_oracle(rtol, atol, oracle, torch_dev):
+    """The thread-per-cell step kernel (option step_mapping 1, H2) on a ghosted,
+    masked block against the oracle and against the wave-per-cell kernel: same
+    CVODE-semantics BDF, so the same step counts up to rounding-flipped cells;
+    tight tolerance: rel 1e-8 in Y where rtol governs and 1e-6 K in T;
+    reference tolerance: within the solver's error weight."""
+    from tiger_nga_amd.finitechem import FiniteChem
+    nx, ny, nz = 8, 8, 8
+    g, RHO_, SC_, mask, P, dt = _grid_case("h2", "h2", nx, ny, nz, 2, 13, True)
+    res = {}
+    for mapping in (1, 0):
+        fc = FiniteChem("h2", rtol, atol)
+        fc.set_option("step_mapping", mapping)
+        S = SC_.copy(order="F")
+        HR = np.full(RHO_.shape, 3.0, order="F")
+        CS = np.full(RHO_.shape + (3,), 1e-60, order="F")
+        st = fc.source_chemistry(RHO_, S, g, dt, P, P, mask=mask, HR_gp=HR, chemsrc_gp=CS)
+        fc.close()
+        res[mapping] = (S, HR, CS, st)
+    SC_o = SC_.copy(order="F")
+    HR_o = np.full(RHO_.shape, 3.0, order="F")
+    CS_o = np.full(RHO_.shape + (3,), 1e-60, order="F")
+    sto = oracle.source_chemistry("h2", RHO_, SC_o, g, dt, P, P, mask=mask, HR_gp=HR_o, chemsrc_gp=CS_o,
+                                  rtol=rtol, atol=atol)
+    S, HR, CS, st = res[1]
+    nsp = len(meta.species("h2"))
+    sp = slice(g.isc_sc - 1, g.isc_sc - 1 + nsp)
+    print("\nh2 thread per cell rtol=%g: steps %d (wave per cell %d, oracle %d), nfe %d, failed %d"
+          % (rtol, st["nst"], res[0][3]["nst"], sto["nst"], st["nfe"], st["failed"]))
+    assert st["failed"] == 0 and st["cells"] == res[0][3]["cells"]
+    assert abs(st["nst"] - sto["nst"]) <= 0.01 * sto["nst"]
+    wall = mask.astype(bool)
+    assert np.array_equal(S[wall], SC_[wall]) and np.all(HR[wall] == 3.0)
+    assert np.array_equal(S[..., 0], SC_[..., 0])
+    rho = RHO_[..., None]
+    eY = _rel_err(S[..., sp], SC_o[..., sp], 1e-12)
+    eT = np.abs(S[..., g.isc_T - 1] - SC_o[..., g.isc_T - 1])
+    ew = np.abs(rho * (S[..., sp] - SC_o[..., sp])) / (rtol * np.abs(rho * SC_o[..., sp]) + atol)
+    hr_rel = (np.abs(HR - HR_o) / np.maximum(np.abs(HR_o), 1.0)).max()
+    print("   vs oracle: max rel Y %.2e, max |dT| %.2e K, max weighted %.2e, HR rel %.2e" % (eY.max(), eT.max(),
+                                                                                          ew.max(), hr_rel))
+    if rtol < 1e-8:
+        rel_dom = rtol * np.abs(rho * SC_o[..., sp]) >= 10.0 * atol
+        assert eY[rel_dom].max() < 1e-8 and eT.max() < 1e-6
+    else:
+        ewc = ew.max(axis=3)
+        assert np.sort(ewc.ravel())[-3] <= 1.0 and ewc.max() <= 10.0 and eT.max() < 1e-2
+    if rtol < 1e-8:
+        assert hr_rel < 1e-6

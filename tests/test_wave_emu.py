@@ -24,6 +24,8 @@ def emu():
     lib.emu_step.argtypes = [ctypes.c_char_p, ctypes.c_long, vp, vp, ctypes.c_double, ctypes.c_double,
                              ctypes.c_double, ctypes.c_double, ctypes.c_int, vp, vp, vp, ctypes.c_int]
     lib.emu_lane_rhs.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_double, vp, vp]
+    lib.emu_lane_step.argtypes = [ctypes.c_char_p, ctypes.c_long, vp, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_double, ctypes.c_int, vp]
     lib.emu_jac.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                             ctypes.c_double, vp, vp, vp, vp, vp, vp]
     return lib
@@ -215,3 +217,27 @@ def test_emulated_kernel_analytic_jacobian_matches_oracle(emu, oracle, workload,
     dom = rtol * rho[:, None] * np.abs(Yo) >= 10 * atol
     assert rel[same][dom[same]].max(initial=0.0) <= 1e-8
     assert np.abs(sc[-1] - ref[:, -1] / rho).max() < 1e-6
+
+
+@pytest.mark.parametrize("rtol,atol", [(1e-6, 1e-12), (1e-10, 1e-16)])
+def test_lane_solver_matches_oracle(emu, oracle, rtol, atol):
+    """The thread-per-cell solver (LaneBdf, bdf_lane.h), compiled for the CPU,
+    against the oracle on H2 cells: the same CVODE-semantics decisions
+    (identical counters on all but rounding-flipped cells) and, where the
+    sequences agree, results within a fraction of the solver's own error weight
+    (its right-hand side is the generated lane code, whose rounding differs from
+    the oracle's table-driven one; Newton stops within 0.1 of the weight)."""
+    mech, Y, T, rho, P, dt = inputs.states("h2", n=24, seed=8)
+    sol = np.ascontiguousarray(inputs.packed_state(Y, T, rho))
+    n = sol.shape[0]
+    cnt = np.zeros((n, 10), dtype=np.int32)
+    got = sol.copy()
+    assert emu.emu_lane_step(b"h2", n, got.ctypes.data, dt, P, rtol, atol, 10000, cnt.ctypes.data) == 0
+    ref, st = oracle.integrate_batch(mech, sol, dt, P, rtol=rtol, atol=atol, with_stats=True)
+    assert (cnt[:, 9] == 0).all() and (st[:, 9] == 0).all()
+    same = (cnt[:, :9] == st[:, :9]).all(axis=1)
+    assert same.sum() >= n - 1, (cnt, st)
+    ew = (np.abs(got[:, :-1] - ref[:, :-1]) / (rtol * np.abs(ref[:, :-1]) + atol)).max(axis=1)
+    assert ew[same].max(initial=0.0) <= 1.0 and ew.max() <= 10.0
+    dT = np.abs(got[:, -1] - ref[:, -1]) / rho
+    assert dT[same].max(initial=0.0) < (1e-6 if rtol < 1e-8 else 1e-3)

@@ -5,6 +5,8 @@ Choices (all exposed as context options of include/tiger_chem.h):
 
 * ``rhs_mapping``  batched right-hand side (tgr_chem_rhs_device): one state
   per lane (generated straight-line code) or one state per wavefront;
+* ``step_mapping`` the plain step: a wavefront per cell or, for the
+  mechanisms small enough (H2), a thread per cell;
 * ``grid``         workgroups of the persistent step kernel: the resident
   capacity (occupancy x CUs) or a fraction of it (fewer workgroups leave CUs
   idle but reduce contention for the L2 and the work counter at small sizes);
@@ -124,13 +126,26 @@ def tune(mech: str, cells: int = 32 ** 3, seed: int = 7, reps: int = 2, log=prin
     rec["grid"] = best_grid
     rec["measured"]["step_ms"] = {str(k): v * 1e3 for k, v in gt.items()}
     fc.set_option("grid", best_grid)
-    # the plain step's Jacobian: difference quotients or analytic
+    # the plain step's mapping: a wavefront per cell or a thread per cell
+    mt = {0: _time(torch, step, reps)}
+    if fc.neq == 10:
+        fc.set_option("step_mapping", 1)
+        mt[1] = _time(torch, step, reps)
+        fc.set_option("step_mapping", 0)
+    for k, v in mt.items():
+        log("  step mapping %s: %.2f ms" % ("thread per cell" if k else "wave per cell", v * 1e3))
+    best_step_map = min(mt, key=mt.get)
+    rec["step_mapping"] = best_step_map
+    rec["measured"]["step_mapping_ms"] = {("thread" if k else "wave"): v * 1e3 for k, v in mt.items()}
+    fc.set_option("step_mapping", best_step_map)
+    # the plain step's Jacobian: difference quotients or analytic (the thread
+    # per cell mapping has difference quotients only)
     jt = {}
     for jac in (0, 1):
         fc.set_option("jacobian", jac)
         jt[jac] = _time(torch, step, reps)
         log("  step jacobian %s: %.2f ms" % ("analytic" if jac else "difference quotients", jt[jac] * 1e3))
-    best_jac = 1 if jt[1] < 0.98 * jt[0] else 0
+    best_jac = 1 if jt[1] < 0.98 * jt[0] and jt[1] < mt[best_step_map] else 0
     rec["jacobian"] = best_jac
     rec["measured"]["step_jacobian_ms"] = {"difference_quotients": jt[0] * 1e3, "analytic": jt[1] * 1e3}
     fc.set_option("jacobian", best_jac)
@@ -245,7 +260,7 @@ def apply(fc, cells: int, path=CACHE) -> dict:
     done = {}
     if not rec:
         return done
-    for k in ("rhs_mapping", "grid", "jacobian", "surr_adaptive", "surr_nsub", "surr_grade", "surr_max_att",
+    for k in ("rhs_mapping", "grid", "step_mapping", "jacobian", "surr_adaptive", "surr_nsub", "surr_grade", "surr_max_att",
               "surr_guess", "surr_h0_fac", "surr_jac_theta", "surr_max_jac", "surr_tol_y"):
         if k in rec:
             fc.set_option(k, rec[k])

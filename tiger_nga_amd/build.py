@@ -3,7 +3,7 @@ ISO_C_BINDING module and the Fortran test caller with flang.
 
 Flags: -O3, C++17, --offload-arch=gfx950, and packed-FP32 instruction
 selection disabled for device code (-Xclang -target-feature -Xclang
--packed-fp32-ops).  Usage: ``python -m tiger_nga_amd.build``.
+-packed-fp32-ops); a use of an uninitialised variable is an error.  Usage: ``python -m tiger_nga_amd.build``.
 """
 import os
 import subprocess
@@ -12,15 +12,15 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libtiger_chem.so")
-SOURCES = ["chem_bdf.hip", "props.hip", "surrogate.hip", "mlp.hip", "noalloc.hip", "chem_capi.cpp",
-           "tables_host.cpp"]
+SOURCES = ["chem_bdf.hip", "chem_lane.hip", "props.hip", "surrogate.hip", "mlp.hip", "noalloc.hip", "diffusion.hip",
+           "chem_capi.cpp", "tables_host.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TGR_OFFLOAD_ARCH", "gfx950")
 
 
 def command(out=OUT, extra=()):
     return [HIPCC, "-O3", "-std=c++17", "--offload-arch=" + ARCH, "-fPIC", "-shared",
-            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Werror=uninitialized",
             "-I" + os.path.join(HERE, "..", "include"), *extra,
             *[os.path.join(CSRC, s) for s in SOURCES], "-o", out]
 

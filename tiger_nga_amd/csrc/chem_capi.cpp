@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -74,6 +75,7 @@ struct tgr_chem {
   double rtol = 1e-6, atol = 1e-12;
   int max_steps = 10000, qmax = 5;
   int jac = 0;  // 0: difference-quotient Jacobian (the reference's CVODE FCVDENSE); 1: analytic
+  int step_mapping = 0;  // 0: a wave per cell; 1: a thread per cell (small mechanisms, bdf_lane.h)
   int rhs_lanes = 1;  // batched right-hand side: one state per lane (1) or per wave (0)
   int64_t grid_override = 0;
   int device = 0;
@@ -99,6 +101,7 @@ struct tgr_chem {
   double* d_out_scale = nullptr;
   std::vector<float> h_in_shift, h_in_scale;
   DevBuf b_x, b_guess, b_rej;
+  DevBuf b_diff;  // diffusion workspace: W_gp, cp_mix, corr_vel x / y / z (5 planes)
   unsigned int* d_rej_count = nullptr;
   unsigned long long* d_stotals = nullptr;
   int surr_nsub = 8, surr_guess = 0, surr_newton_max = 8;
@@ -254,9 +257,17 @@ int run_step(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const
   StepParams p = step_params(c, dt, p_thermo, p_thermo_old, g, rho, sc, isc_sc0, isc_T0, mask, hr, chem);
   p.cell_list = list;
   p.nlist = nlist;
-  const int grid = persistent_grid(c, tgr_step_occupancy(c->neq), p.ncell, tgr_step_block_waves());
+  // a thread per cell: one 64-thread workgroup per wave slot, each thread
+  // takes cells one by one (persistent grid of the resident capacity)
+  const bool lanes = c->step_mapping == 1 && tgr_step_lane_supported(c->neq) && !c->jac;
+  const int grid = lanes ? (int)std::max<int64_t>(1, std::min<int64_t>(
+                               c->grid_override > 0 ? c->grid_override
+                                                    : (int64_t)c->n_cu * std::max(1, tgr_step_lane_occupancy(c->neq)),
+                               (p.ncell + 63) / 64))
+                         : persistent_grid(c, tgr_step_occupancy(c->neq), p.ncell, tgr_step_block_waves());
   const int wpb = tgr_step_block_waves();
-  const size_t ws_bytes = (size_t)grid * wpb * c->neq * 64 * sizeof(double);
+  const size_t ws_bytes = lanes ? (size_t)grid * tgr_step_lane_ws(c->neq) * sizeof(double)
+                                : (size_t)grid * wpb * c->neq * 64 * sizeof(double);
   if (scratch) {
     if (ws_bytes > scratch_bytes - kScrWs)
       return fail(TGR_E_ARG, "the step's workspace (" + std::to_string(ws_bytes) + " bytes for a grid of " +
@@ -275,7 +286,7 @@ int run_step(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const
   // with stats == NULL the call enqueues two kernels and nothing else (no
   // memset, no event), so that it can be captured in a HIP graph
   if (stats) HIP_TRY(hipEventRecord(c->ev0, s));
-  int rc = tgr_launch_step(&p, c->neq, grid, s);
+  int rc = lanes ? tgr_launch_step_lane(&p, c->neq, grid, s) : tgr_launch_step(&p, c->neq, grid, s);
   if (rc) return fail(TGR_E_HIP, "step kernel launch failed");
   if (stats) HIP_TRY(hipEventRecord(c->ev1, s));
   if (stats) {
@@ -367,6 +378,7 @@ int tgr_chem_destroy(tgr_chem* c) {
   c->b_x.release();
   c->b_guess.release();
   c->b_rej.release();
+  c->b_diff.release();
   c->b_ws.release();
   c->b_rho.release();
   c->b_sc.release();
@@ -401,6 +413,9 @@ int tgr_chem_set_option(tgr_chem* c, const char* key, int64_t v) {
   else if (!strcmp(key, "rhs_mapping")) {
     if (v != 0 && v != 1) return fail(TGR_E_ARG, "rhs_mapping must be 0 (wave per state) or 1 (lane per state)");
     c->rhs_lanes = (int)v;
+  } else if (!strcmp(key, "step_mapping")) {
+    if (v != 0 && v != 1) return fail(TGR_E_ARG, "step_mapping must be 0 (wave per cell) or 1 (thread per cell)");
+    c->step_mapping = (int)v;
   } else if (!strcmp(key, "jacobian")) {
     if (v != 0 && v != 1) return fail(TGR_E_ARG, "jacobian must be 0 (difference quotients) or 1 (analytic)");
     c->jac = (int)v;
@@ -595,7 +610,7 @@ int run_surrogate(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, 
   // accepted when that correction -- the distance of the prediction from the
   // stage equation's solution -- is within surr_sa_tol acceptance tolerances
   const bool standalone = c->surr_mode == 1;
-  const int adaptive = standalone ? 0 : adaptive;
+  const int adaptive = standalone ? 0 : c->surr_adaptive;
   const int64_t rows = (p.ncell + 63) / 64 * 64;
   // network output times: every stage of the fixed schedule, or the end of
   // each sub-step of the schedule for the adaptive solve (which interpolates)
@@ -895,6 +910,59 @@ int tgr_chem_properties_device(tgr_chem* c, double p_thermo, const tgr_grid* g, 
   p.visc = VISC_;
   p.diff = DIFF_;
   if (tgr_launch_props(&p, c->neq, (hipStream_t)stream)) return fail(TGR_E_HIP, "launch");
+  return 0;
+}
+
+int tgr_chem_diffusion_device(tgr_chem* c, double dt, const tgr_grid* g, const tgr_metric* m, int flags,
+                              const double* RHO_, const double* SC_, const double* DIFF_, const int* mask,
+                              double* diff_corr, double* diff_gp, double* src, void* stream) {
+  if (!c || !m || !RHO_ || !SC_ || !DIFF_) return fail(TGR_E_ARG, "NULL argument");
+  int rc = check_grid(g, c->nsp);
+  if (rc) return rc;
+  if (!(dt > 0.0) || !std::isfinite(dt)) return fail(TGR_E_ARG, "dt must be positive and finite");
+  if (m->st1 < 0 || m->st2 < 1 || m->st1 + m->st2 > 8) return fail(TGR_E_ARG, "stencil widths out of range");
+  const int w = m->st1 + m->st2;
+  if (g->imin_ - w < g->imino_ || g->imax_ + w > g->imaxo_ || g->jmin_ - w < g->jmino_ || g->jmax_ + w > g->jmaxo_ ||
+      g->kmin_ - w < g->kmino_ || g->kmax_ + w > g->kmaxo_)
+    return fail(TGR_E_ARG, "the block needs st1 + st2 ghost cells on every side");
+  if (mask && (g->imino > g->imino_ || g->imaxo < g->imaxo_ || g->jmino > g->jmino_ || g->jmaxo < g->jmaxo_))
+    return fail(TGR_E_ARG, "mask must cover the array extents");
+  static_assert(sizeof(tgr_metric) == sizeof(tgr::tgr_metric_dev) &&
+                    offsetof(tgr_metric, grad_zm) == offsetof(tgr::tgr_metric_dev, grad_zm),
+                "tgr_metric and its device view must share a layout");
+  const tgr::tgr_metric_dev* md = (const tgr::tgr_metric_dev*)m;
+  const double* const* ops = &md->interp_sc_x;
+  for (int q = 0; q < 15; ++q)
+    if (!ops[q]) return fail(TGR_E_ARG, "a metric operator is NULL");
+  tgr::DiffParams p{};
+  tgr::DiffGeom& gm = p.gm;
+  gm.imin_ = g->imin_, gm.imax_ = g->imax_, gm.jmin_ = g->jmin_, gm.jmax_ = g->jmax_, gm.kmin_ = g->kmin_,
+  gm.kmax_ = g->kmax_;
+  gm.imino_ = g->imino_, gm.jmino_ = g->jmino_, gm.kmino_ = g->kmino_;
+  gm.nxo = g->imaxo_ - g->imino_ + 1, gm.nyo = g->jmaxo_ - g->jmino_ + 1, gm.nzo = g->kmaxo_ - g->kmino_ + 1;
+  gm.imino = g->imino, gm.jmino = g->jmino, gm.mx = g->imaxo - g->imino + 1;
+  gm.plane = (int64_t)gm.nxo * gm.nyo * gm.nzo;
+  gm.isc_sc0 = g->isc_sc - 1, gm.isc_T0 = g->isc_T - 1;
+  p.m = *md;
+  p.mech = c->dm;
+  p.dt = dt;
+  p.molar = (flags & TGR_DIFF_MOLAR) != 0;
+  p.gasrad = (flags & TGR_DIFF_GASRAD) != 0;
+  p.cv = (flags & TGR_DIFF_CV) != 0;
+  p.rho = RHO_, p.sc = SC_, p.diff = DIFF_, p.mask = mask;
+  p.corr = diff_corr, p.dgp = diff_gp, p.src = src;
+  const char* names[4] = {"H2O", "CO", "CO2", "CH4"};
+  int* idx[4] = {&p.i_h2o, &p.i_co, &p.i_co2, &p.i_ch4};
+  double* wk[4] = {&p.w_h2o, &p.w_co, &p.w_co2, &p.w_ch4};
+  for (int q = 0; q < 4; ++q) {
+    *idx[q] = species_index(c->t, names[q]);
+    *wk[q] = *idx[q] >= 0 ? c->t->W[*idx[q]] : 1.0;
+  }
+  if ((rc = c->b_diff.ensure((size_t)gm.plane * 5 * sizeof(double)))) return rc;
+  p.wg = (double*)c->b_diff.p;
+  p.cpm = p.wg + gm.plane;
+  p.cvel = p.cpm + gm.plane;
+  if (tgr_launch_diffusion(&p, c->neq, (hipStream_t)stream)) return fail(TGR_E_HIP, "diffusion kernel launch failed");
   return 0;
 }
 

@@ -25,6 +25,13 @@ int tgr_launch_surr_features(const tgr::StepParams* p, int neq, int64_t rows, in
                              const float* scale, float log_dt, uint16_t* X, hipStream_t s);
 int tgr_launch_surr_set_column(uint16_t* X, int64_t rows, int in_pad, int col, float value, hipStream_t s);
 int tgr_launch_surr(const tgr::SurrParams* P, int neq, int grid, hipStream_t s);
+int tgr_launch_diffusion(const tgr::DiffParams* p, int neq, hipStream_t s);
+// the thread-per-cell step kernel (chem_lane.hip): supported sizes, workspace
+// doubles per workgroup, workgroups per CU, launch
+int tgr_step_lane_supported(int neq);
+int64_t tgr_step_lane_ws(int neq);
+int tgr_step_lane_occupancy(int neq);
+int tgr_launch_step_lane(const tgr::StepParams* p, int neq, int grid, hipStream_t s);
 // device address and size of the non-allocating entry point's scratch (noalloc.hip)
 int tgr_scratch_resolve(int device, void** ptr, size_t* bytes);
 }

@@ -83,6 +83,47 @@ struct SurrParams {
 // rejected for Newton failure, for the error estimate, for the enthalpy check
 constexpr int kSurrNCount = 8;
 
+// Geometry of a ghosted block for the diffusion stencils (Fortran indices as
+// in tgr_grid; plane = nxo * nyo * nzo; 0-based scalar planes).
+struct DiffGeom {
+  int imin_, imax_, jmin_, jmax_, kmin_, kmax_;
+  int imino_, jmino_, kmino_, nxo, nyo, nzo;
+  int imino, jmino, mx;  // mask(imino:, jmino:), leading dimension mx
+  int64_t plane;
+  int isc_sc0, isc_T0;
+};
+
+// Metric operator arrays (device pointers) of tgr_metric.
+struct tgr_metric_dev {
+  int st1, st2;
+  const double *interp_sc_x, *interp_sc_y, *interp_sc_z;
+  const double *grad_x, *grad_y, *grad_z;
+  const double *div_u, *div_v, *div_w;
+  const double *interp_u_xm, *interp_v_ym, *interp_w_zm;
+  const double *grad_xm, *grad_ym, *grad_zm;
+};
+
+// One launch of the three kernels (filled by the C-ABI layer).
+struct DiffParams {
+  DiffGeom gm;
+  tgr_metric_dev m;
+  DevMech mech;
+  double dt;
+  int molar, gasrad, cv;
+  const double* rho;
+  const double* sc;
+  const double* diff;
+  const int* mask;
+  double* wg;     // [plane] W_gp
+  double* cpm;    // [plane] cp_mix (cv)
+  double* cvel;   // [3][plane] corr_vel_x, _y, _z
+  double* corr;   // diff_corr(..., nsp + 1) or null
+  double* dgp;    // diff_gp(..., 3) or null
+  double* src;    // the transport source (..., nscalar) or null
+  int i_h2o, i_co, i_co2, i_ch4;  // species indices (-1: absent)
+  double w_h2o, w_co, w_co2, w_ch4;
+};
+
 // Transport / equation-of-state properties of a whole Fortran-layout block.
 // ptab: mucoeff[64], kOverEps[64], W[64], then A[nsp][64] and B[nsp][64]
 // (pair factors, row j, lane k).

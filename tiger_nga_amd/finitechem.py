@@ -50,6 +50,14 @@ class _SurrStats(ctypes.Structure):
         ("surrogate_ms", ctypes.c_double), ("fallback_ms", ctypes.c_double), ("guided", ctypes.c_int)]
 
 
+METRIC_OPERATORS = ("interp_sc_x", "interp_sc_y", "interp_sc_z", "grad_x", "grad_y", "grad_z", "div_u", "div_v",
+                    "div_w", "interp_u_xm", "interp_v_ym", "interp_w_zm", "grad_xm", "grad_ym", "grad_zm")
+
+
+class _Metric(ctypes.Structure):
+    _fields_ = [("st1", ctypes.c_int), ("st2", ctypes.c_int)] + [(n, ctypes.c_void_p) for n in METRIC_OPERATORS]
+
+
 class _Info(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
         "nsp", "nrxn", "nslot", "n_third", "n_falloff", "n_third_body_rxn", "n_reversible", "n_nu", "n_fc_terms",
@@ -144,6 +152,8 @@ def load_library(path: str = LIB_PATH):
     lib.tgr_chem_temperature_device.argtypes = [vp, ctypes.c_int64, vp, vp, vp, vp, vp, vp]
     lib.tgr_chem_properties_device.argtypes = [vp, ctypes.c_double, ctypes.POINTER(_Grid), vp, vp, vp, ctypes.c_int,
                                                vp, vp, vp, vp]
+    lib.tgr_chem_diffusion_device.argtypes = [vp, ctypes.c_double, ctypes.POINTER(_Grid), ctypes.POINTER(_Metric),
+                                              ctypes.c_int, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.tgr_chem_set_option.argtypes = [vp, ctypes.c_char_p, ctypes.c_int64]
     lib.tgr_chem_set_option_real.argtypes = [vp, ctypes.c_char_p, ctypes.c_double]
     fp = ctypes.POINTER(ctypes.c_float)
@@ -355,6 +365,20 @@ class FiniteChem:
         _check(self.lib, self.lib.tgr_chem_properties_device(
             self._h, p_thermo, ctypes.byref(g), _ptr(SC_), _ptr(mask), _ptr(Lewis), int(isc_zmix), _ptr(RHO_),
             _ptr(VISC_), _ptr(DIFF_), stream), "properties")
+
+    def diffusion(self, dt: float, grid: Grid, metric: dict, RHO_, SC_, DIFF_, mask=None, diff_corr=None,
+                  diff_gp=None, src=None, molar: bool = True, gasrad: bool = False, cv: bool = False, stream=None):
+        """finitechem_diffusion (finitechem.f90:538-923) on device tensors in
+        the step's Fortran layout; metric: st1, st2 and the 15 operator
+        tensors of tgr_metric (Fortran-ordered, the reference's extents, see
+        tiger_nga_amd.metric).  Writes the interior of diff_corr (..., nsp + 1),
+        diff_gp (..., 3) and adds to src (..., nscalar) where given."""
+        m = _Metric(int(metric["st1"]), int(metric["st2"]), *[_ptr(metric[n]) for n in METRIC_OPERATORS])
+        flags = (1 if molar else 0) | (2 if gasrad else 0) | (4 if cv else 0)
+        g = grid._c()
+        _check(self.lib, self.lib.tgr_chem_diffusion_device(
+            self._h, dt, ctypes.byref(g), ctypes.byref(m), flags, _ptr(RHO_), _ptr(SC_), _ptr(DIFF_), _ptr(mask),
+            _ptr(diff_corr), _ptr(diff_gp), _ptr(src), stream), "diffusion")
 
     def temperature(self, Y, hmix, T0=None, stream=None):
         """(T[n], iters[n]) from mass fractions Y[nsp, n] and mixture enthalpy
