@@ -250,7 +250,8 @@ def bench_surrogate(a, torch, fc, g, R_d, S0_d, S_d, HR_d, CS_d, dt, P, mech, sp
     r_max = float((S_d[..., :nsp] - Rp[..., :nsp]).abs().max())
     fc.set_option("surr_probe", 8)
     fc.set_surrogate(None, None, None, None)
-    gf = (flops.guided_flops(fc.info, ss, split=bool(fc.options.get("surr_split_jac", 1)))
+    gf = (flops.guided_flops(fc.info, ss, split=bool(fc.options.get("surr_split_jac", 1))
+                             and not int(fc.options.get("jacobian", 0)))
           if ss["surrogate_ms"] > 0 else 0.0)
 
     def net_case(t, st):
@@ -278,7 +279,9 @@ def bench_surrogate(a, torch, fc, g, R_d, S0_d, S_d, HR_d, CS_d, dt, P, mech, sp
                                        "enthalpy": ss["rej_enthalpy"]},
                           "guided_ms": ss["surrogate_ms"], "fallback_ms": ss["fallback_ms"],
                           "newton_iters_per_cell": ss["newton_iters"] / max(1, cells),
-                          "substeps_per_cell": ss["substeps"] / max(1, cells)},
+                          "substeps_per_cell": ss["substeps"] / max(1, cells),
+                          "factorizations_per_cell": ss["factorizations"] / max(1, cells),
+                          "jacobians_per_cell": ss["jacobians"] / max(1, cells)},
         "options": opts,
         "network": {"used": bool(guess_t), "with_network_guess": with_net, "without_network_guess": without_net,
                     "gain": without_net["ms_per_step"] / with_net["ms_per_step"],

@@ -232,7 +232,9 @@ typedef struct {
   int64_t accepted, rejected;     /* cells accepted by the guided solve / handed to the plain step */
   int64_t newton_iters, rhs_evals; /* of the guided solve (Jacobian columns included) */
   int64_t rej_newton, rej_error, rej_enthalpy; /* why cells were rejected */
-  int64_t substeps;               /* attempted sub-steps of the guided solve (one LU each) */
+  int64_t substeps;               /* sub-steps taken by the guided solve (accepted ones, until a cell stops) */
+  int64_t factorizations;         /* LU factorisations of I - gamma h J in the guided solve */
+  int64_t jacobians;              /* Jacobian evaluations of the guided solve (1 per cell, more with refreshes) */
   double surrogate_ms, fallback_ms; /* device time of features + network + guided solve, of the plain step */
   int guided;                     /* 1: the guided solve ran; 0: the adaptive policy sent the call to the plain step */
 } tgr_surrogate_stats;

@@ -18,9 +18,9 @@ def test_tune_and_apply_h2(tmp_path):
     if "surr_nsub" in rec:
         assert 1 <= rec["surr_nsub"] <= 8 and rec["surr_grade"] >= 1.0
         assert rec["surr_adaptive"] in (0, 1) and rec["surr_guess"] in (0, 1) and rec["surr_tol_y"] in autotune.SURR_TOLS
-        name = "%s n%d r%.1f a%d guess%d h0f%g tol%.1e" % (
-            "adaptive" if rec["surr_adaptive"] else "fixed", rec["surr_nsub"], rec["surr_grade"], rec["surr_max_att"],
-            rec["surr_guess"], rec["surr_h0_fac"], rec["surr_tol_y"])
+        name = autotune.surr_name(rec["surr_adaptive"], rec["surr_nsub"], rec["surr_grade"], rec["surr_max_att"],
+                                  rec["surr_guess"], rec["surr_h0_fac"], rec["surr_max_jac"], rec["surr_keep"],
+                                  rec["surr_tol_y"])
         m = rec["measured"]["surrogate"][name]
         assert max(m["max_species_err_generated"], m["max_species_err_relaxed"]) <= 5e-7
     cache = str(tmp_path / "tuning.json")

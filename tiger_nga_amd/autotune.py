@@ -54,7 +54,10 @@ SURR_SCHEDULES = ((1, 8, 2.0, 48), (1, 4, 1.0, 32), (1, 8, 3.0, 64), (0, 4, 1.0,
 # mechanisms that take many need the smaller ones
 SURR_TOLS = (2.5e-7, 5e-7, 1e-6, 2e-6)
 # first step of the adaptive schedule from the initial rate (0: the schedule's)
-SURR_H0 = (0.0, 4.0, 16.0)
+SURR_H0 = (0.0, 16.0)
+# adaptive schedule: growth factors in [1, keep) keep the step and its LU
+# factors (1.0: every accepted step takes its new size)
+SURR_KEEP = (1.0, 1.5)
 # Jacobians per cell of the adaptive schedule (updates when Newton contracts
 # slower than SURR_JAC_THETA with a Jacobian of an earlier state)
 SURR_MAX_JAC = (1, 4)
@@ -186,11 +189,13 @@ def tune(mech: str, cells: int = 32 ** 3, seed: int = 7, reps: int = 2, log=prin
                 # the schedule's first sub-step)
                 for h0f in (SURR_H0 if adaptive and not guess else (0.0,)):
                     for mj in (SURR_MAX_JAC if adaptive else (1,)):
-                        combos.append((adaptive, nsub, grade, att, guess, h0f, mj))
+                        for keep in (SURR_KEEP if adaptive and not guess else (1.5,)):
+                            combos.append((adaptive, nsub, grade, att, guess, h0f, mj, keep))
         fc.set_option("surr_jac_theta", SURR_JAC_THETA)
-        for (adaptive, nsub, grade, att, guess, h0f, mj) in combos:
+        for (adaptive, nsub, grade, att, guess, h0f, mj, keep) in combos:
             for tol in SURR_TOLS:
                 fc.set_option("surr_max_jac", mj)
+                fc.set_option("surr_keep", keep)
                 fc.set_option("surr_adaptive", adaptive)
                 fc.set_option("surr_nsub", nsub)
                 fc.set_option("surr_grade", grade)
@@ -210,24 +215,29 @@ def tune(mech: str, cells: int = 32 ** 3, seed: int = 7, reps: int = 2, log=prin
                     torch.cuda.synchronize()
                     errs.append(float((Sb[..., :nsp] - ref[..., :nsp]).abs().max()))
                     terrs.append(float((Sb[..., nsp] - ref[..., nsp]).abs().max()))
-                name = "%s n%d r%.1f a%d guess%d h0f%g jac%d tol%.1e" % (
-                    "adaptive" if adaptive else "fixed", nsub, grade, att, guess, h0f, mj, tol)
+                name = surr_name(adaptive, nsub, grade, att, guess, h0f, mj, keep, tol)
                 st_all[name] = {"ms_generated_plus_relaxed": t_tot * 1e3, "max_species_err_generated": errs[0],
                                 "max_species_err_relaxed": errs[1], "max_T_err_K": max(terrs)}
                 log("  surrogate %s: %.2f ms, max |dY| %.2e / %.2e, max |dT| %.2e K" % (
                     name, t_tot * 1e3, errs[0], errs[1], max(terrs)))
                 # half the 1e-6 bound: larger blocks reach further into the tail
                 if max(errs) <= MAX_SPECIES_ERR and max(terrs) <= MAX_T_ERR and (best is None or t_tot < best[0]):
-                    best = (t_tot, adaptive, nsub, grade, att, guess, h0f, mj, tol)
+                    best = (t_tot, adaptive, nsub, grade, att, guess, h0f, mj, keep, tol)
         rec["measured"]["surrogate"] = st_all
         if best:
             (_, rec["surr_adaptive"], rec["surr_nsub"], rec["surr_grade"], rec["surr_max_att"], rec["surr_guess"],
-             rec["surr_h0_fac"], rec["surr_max_jac"], rec["surr_tol_y"]) = best
+             rec["surr_h0_fac"], rec["surr_max_jac"], rec["surr_keep"], rec["surr_tol_y"]) = best
             rec["surr_jac_theta"] = SURR_JAC_THETA
             rec["measured"]["surrogate_speedup"] = (sets[0][5] + sets[1][5]) / best[0]
         fc.set_surrogate(None, None, None, None)
     fc.close()
     return rec
+
+
+def surr_name(adaptive, nsub, grade, att, guess, h0f, mj, keep, tol):
+    """Key of a guided-solve candidate in a record's measurements."""
+    return "%s n%d r%.1f a%d guess%d h0f%g jac%d keep%.1f tol%.1e" % (
+        "adaptive" if adaptive else "fixed", nsub, grade, att, guess, h0f, mj, keep, tol)
 
 
 def load_cache(path=CACHE):
@@ -261,7 +271,7 @@ def apply(fc, cells: int, path=CACHE) -> dict:
     if not rec:
         return done
     for k in ("rhs_mapping", "grid", "step_mapping", "jacobian", "surr_adaptive", "surr_nsub", "surr_grade", "surr_max_att",
-              "surr_guess", "surr_h0_fac", "surr_jac_theta", "surr_max_jac", "surr_tol_y"):
+              "surr_guess", "surr_h0_fac", "surr_jac_theta", "surr_max_jac", "surr_keep", "surr_tol_y"):
         if k in rec:
             fc.set_option(k, rec[k])
             done[k] = rec[k]

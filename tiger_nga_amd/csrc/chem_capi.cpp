@@ -743,7 +743,9 @@ int run_surrogate(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, 
       sstats->rej_newton = st[4];
       sstats->rej_error = st[5];
       sstats->rej_enthalpy = st[6];
-      sstats->substeps = st[7];
+      sstats->substeps = st[8];
+      sstats->factorizations = st[7];
+      sstats->jacobians = st[9];
       sstats->surrogate_ms = ms_s;
       sstats->fallback_ms = ms_p;
     }

@@ -81,7 +81,7 @@ struct SurrParams {
 };
 // surrogate counters: accepted, rejected, Newton iterations, RHS evaluations,
 // rejected for Newton failure, for the error estimate, for the enthalpy check
-constexpr int kSurrNCount = 8;
+constexpr int kSurrNCount = 10;
 
 // Geometry of a ghosted block for the diffusion stencils (Fortran indices as
 // in tgr_grid; plane = nxo * nyo * nzo; 0-based scalar planes).

@@ -92,7 +92,7 @@ __global__ void surr_reset_kernel(unsigned int* queue, unsigned int* rej_count, 
 template <int NEQ, bool AD, bool NET, bool AJ>
 __device__ __forceinline__ int surr_cell(const SurrParams& P, const DevMech& m, tchar* tab, TGR_LDS RhsLds<NEQ>* rl,
                                          TGR_LDS RateCache<NEQ>* rc, int64_t cell, double* ws,
-                                         int& nit, int& nfe, int& nlu) {
+                                         int& nit, int& nfe, int& nlu, int& nsub, int& nje) {
   const StepParams& p = P.s;
   const int lane = lane_id();
   const int nsp = NEQ - 1;
@@ -139,6 +139,8 @@ __device__ __forceinline__ int surr_cell(const SurrParams& P, const DevMech& m, 
   }
   nfe = b.c_nfe + b.c_nfe_dq;
   nlu = b.c_nlu;
+  nsub = b.nst;
+  nje = b.c_nje;
   if (st) return st;
   // element-conserving projection: the least change of the species (weights
   // |rho Y_k|) that restores every element's mass to its initial value
@@ -263,9 +265,9 @@ __global__ __launch_bounds__(kSurrBlock) void surr_kernel(SurrParams P) {
     if (lane == 0) c = atomicAdd(P.s.queue, 1u);
     const int64_t cell = (int64_t)(unsigned int)__builtin_amdgcn_readfirstlane((int)c);
     if (cell >= P.s.ncell) break;
-    int nit = 0, nfe = 0, nlu = 0;
+    int nit = 0, nfe = 0, nlu = 0, nsub = 0, nje = 0;
     const int st = surr_cell<NEQ, AD, NET, AJ>(P, m, (tchar*)tab_s, (TGR_LDS RhsLds<NEQ>*)&rl[w],
-                                               (TGR_LDS RateCache<NEQ>*)&rcs[w], cell, ws, nit, nfe, nlu);
+                                               (TGR_LDS RateCache<NEQ>*)&rcs[w], cell, ws, nit, nfe, nlu, nsub, nje);
     if (st < 0) continue;
     if (lane == 0) {
       if (st != 0) {
@@ -277,6 +279,8 @@ __global__ __launch_bounds__(kSurrBlock) void surr_kernel(SurrParams P) {
       atomicAdd(&P.stotals[3], (unsigned long long)nfe);
       if (st > 0) atomicAdd(&P.stotals[3 + st], 1ull);
       atomicAdd(&P.stotals[7], (unsigned long long)nlu);
+      atomicAdd(&P.stotals[8], (unsigned long long)nsub);
+      atomicAdd(&P.stotals[9], (unsigned long long)nje);
     }
   }
 }

@@ -46,7 +46,7 @@ REAL_OPTIONS = ("surr_tol_y", "surr_tol_t", "surr_newton_tol", "surr_grade", "su
 class _SurrStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
         "accepted", "rejected", "newton_iters", "rhs_evals", "rej_newton", "rej_error", "rej_enthalpy",
-        "substeps")] + [
+        "substeps", "factorizations", "jacobians")] + [
         ("surrogate_ms", ctypes.c_double), ("fallback_ms", ctypes.c_double), ("guided", ctypes.c_int)]
 
 

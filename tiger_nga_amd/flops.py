@@ -82,19 +82,22 @@ def properties_flops(nsp: int) -> float:
 def guided_flops(info: dict, sstats: dict, nsub: int = 0, split: bool = False) -> float:
     """Algorithmic flops of the surrogate step's guided solve (before the plain
     fallback) from its counters: every right-hand side (Newton and
-    difference-quotient columns), one LU of I - gamma h J and one filtered
-    error estimate per attempted sub-step (the counted ones; without that
-    counter, nsub per attempted cell), a triangular solve and update per
+    difference-quotient columns), each LU of I - gamma h J, one filtered
+    error estimate per sub-step taken, a triangular solve and update per
     Newton iteration, and the enthalpy check / element projection (taken as
-    60 neq per cell).  split: the Jacobian's nsp species columns per cell are
-    fixed-temperature evaluations (rhs_flops_cached)."""
+    60 neq per cell).  Without the factorisation counter, one LU and one
+    estimate per counted sub-step (or nsub per attempted cell).  split: each
+    Jacobian's nsp species columns are fixed-temperature evaluations
+    (rhs_flops_cached)."""
     n = info["nsp"] + 1
     cells = sstats["accepted"] + sstats["rejected"]
     f_lu = 2.0 * n * n + 2.0 / 3.0 * n ** 3
     f_solve = 2.0 * n * n + 4.0 * n
     subs = sstats.get("substeps") or cells * nsub
-    f = (sstats["rhs_evals"] * rhs_flops(info) + subs * (f_lu + f_solve)
+    lus = sstats.get("factorizations") or subs
+    jacs = sstats.get("jacobians") or cells
+    f = (sstats["rhs_evals"] * rhs_flops(info) + lus * f_lu + subs * f_solve
          + sstats["newton_iters"] * f_solve + cells * 60.0 * n)
     if split:
-        f -= cells * info["nsp"] * (rhs_flops(info) - rhs_flops_cached(info))
+        f -= jacs * info["nsp"] * (rhs_flops(info) - rhs_flops_cached(info))
     return f
