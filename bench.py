@@ -50,9 +50,10 @@ def parse():
     ap.add_argument("--cpu-cells", type=int, default=131072)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all usable host cores, at most 16")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--kernel", default="step", choices=["step", "properties"],
+    ap.add_argument("--kernel", default="step", choices=["step", "properties", "diffusion"],
                     help="step: the chemistry step (the headline); properties: density / viscosity / "
-                         "diffusivities of the same block")
+                         "diffusivities of the same block; diffusion: the molar-diffusion correction "
+                         "(finitechem_diffusion, second order, molar term on) of the same block")
     ap.add_argument("--data-file", default=None,
                     help="step the fields of an NGA data file instead of the synthetic workload "
                          "(needs --mech; no CPU baseline)")
@@ -369,6 +370,93 @@ def bench_properties(a, torch, fc, g, S_d, R_d, P, n, cells_label, mech, sp):
     return 0
 
 
+def bench_diffusion(a, torch, fc, mech, Y, T, rho, P, dt, nx, sp):
+    """finitechem_diffusion (finitechem.f90:538-923) on the block with one ghost
+    layer (second-order operators of a uniform Cartesian mesh, molar term on,
+    diffusivities from the properties kernel); HBM roofline at the algorithmic
+    bytes (tiger_nga_amd/flops.py diffusion_bytes), HIP events on the launch
+    stream; CPU baseline: the C oracle on a 32^3 block of the same states."""
+    import numpy as np
+
+    from tiger_nga_amd import flops, inputs, metric
+    from tiger_nga_amd.finitechem import Grid
+    n = nx ** 3
+    RHO_, SC_, isc_sc, isc_T = inputs.fortran_fields(Y, T, rho, nx, nx, nx, nover=1)
+    g = Grid.block(nx, nx, nx, SC_.shape[3], isc_sc, isc_T, nover=1)
+    R_d = fortran_empty(torch, RHO_.shape, "cuda:0")
+    S_d = fortran_empty(torch, SC_.shape, "cuda:0")
+    R_d.copy_(torch.from_numpy(RHO_))
+    S_d.copy_(torch.from_numpy(SC_))
+    del RHO_, SC_
+    D_d = torch.empty_like(S_d)
+    fc.properties(S_d, g, P, DIFF_=D_d, stream=sp)
+    dx = 1e-4
+    faces = dx * np.arange(nx + 3, dtype=np.float64)
+    m = metric.cartesian(g, faces, faces, dx, 2)
+    md = {k: (v if k in ("st1", "st2") else torch.from_numpy(np.ascontiguousarray(v.ravel(order="F"))).cuda())
+          for k, v in m.items()}
+    nsp = fc.nsp
+    dc = fortran_empty(torch, tuple(R_d.shape) + (nsp + 1,), "cuda:0")
+    gp = fortran_empty(torch, tuple(R_d.shape) + (3,), "cuda:0")
+    _progress("diffusion, %d^3 = %d cells, %d + %d calls" % (nx, n, a.warmup, a.steps))
+
+    def call():
+        fc.diffusion(dt, g, md, R_d, S_d, D_d, diff_corr=dc, diff_gp=gp, molar=True, stream=sp)
+    for _ in range(max(1, a.warmup)):
+        call()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(a.steps):
+        call()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ks = e0.elapsed_time(e1) / 1e3 / a.steps
+    B = flops.diffusion_bytes(nsp, n)
+    out = {"metric": "finitechem_diffusion cell-updates/s, 1 GPU", "value": n * a.steps / el, "unit": "cells/s",
+           "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": el / a.steps * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "%s molar-diffusion correction, second order, %d^3 = %d cells, one ghost layer"
+                                  % (mech, nx, n)},
+           "roofline": {"bound": "hbm", "achieved": B / ks / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                        "frac": B / ks / PEAK_HBM, "traffic": None, "kernel_ms": ks * 1e3,
+                        "algorithmic_bytes_per_call": B,
+                        "kernel": "tgr::diff_cell_kernel + diff_face_kernel + diff_main_kernel (one call)"}}
+    tr = traffic_record(a.workload, "tgr::diff_main_kernel")
+    if tr:
+        out["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
+        out["roofline"]["traffic_source"] = tr["source"]
+    if not a.no_cpu:
+        from oracle import pyoracle
+        nc = 32
+        Rc, Sc, isc_sc, isc_T = inputs.fortran_fields(Y[:nc ** 3], T[:nc ** 3], rho[:nc ** 3], nc, nc, nc, nover=1)
+        gc = Grid.block(nc, nc, nc, Sc.shape[3], isc_sc, isc_T, nover=1)
+        Sc_d = fortran_empty(torch, Sc.shape, "cuda:0")
+        Sc_d.copy_(torch.from_numpy(Sc))
+        Dc_d = torch.empty_like(Sc_d)
+        fc.properties(Sc_d, gc, P, DIFF_=Dc_d, stream=sp)
+        torch.cuda.synchronize()
+        Dc = np.asfortranarray(Dc_d.cpu().numpy())
+        fcs = dx * np.arange(nc + 3, dtype=np.float64)
+        mc = metric.cartesian(gc, fcs, fcs, dx, 2)
+        t0 = time.perf_counter()
+        reps = 0
+        while True:
+            pyoracle.diffusion(mech, gc, mc, dt, Rc, Sc, Dc, molar=True)
+            reps += 1
+            if time.perf_counter() - t0 > 5.0:
+                break
+        cel = (time.perf_counter() - t0) / reps
+        out["cpu_baseline"] = {"value": nc ** 3 / cel, "unit": "cells/s", "cores": 1, "kind": "port",
+                               "sample": "C oracle (oracle/diffusion.c) on a 32^3 block of the same states, "
+                                         "one core, %d calls" % reps}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     a = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -428,6 +516,13 @@ def main():
     sp = stream.cuda_stream
     if a.kernel == "properties":
         return bench_properties(a, torch, fc, g, S0_d, R_d, P, n, cells_label, mech, sp)
+    if a.kernel == "diffusion":
+        if a.data_file:
+            print("--kernel diffusion runs on the synthetic workload only", file=sys.stderr)
+            return 2
+        _, Y, T, rho, _, _ = inputs.states(a.workload, seed=0)
+        del R_d, S0_d, S_d, HR_d, CS_d
+        return bench_diffusion(a, torch, fc, mech, Y, T, rho, P, dt, nx, sp)
     nsp = fc.nsp
     have_guided = not a.no_surrogate and os.path.exists(
         __import__("tiger_nga_amd.surrogate", fromlist=["model_path"]).model_path(mech))

@@ -308,6 +308,11 @@ int tgr_chem_set_option_real(tgr_chem *ctx, const char *key, double value);
  *                 and its iteration matrix in LDS -- for the mechanisms small
  *                 enough for it (H2, neq 10; others, and "jacobian" 1, keep the
  *                 wave per cell); the autotuner times both
+ *   "step_waves"  wave per cell: resident waves per SIMD, 2 (default; up to 256
+ *                 VGPRs per lane) or 4 (at most 128, two workgroups per CU; for
+ *                 neq 10 only, whose tables leave LDS room for the second; with
+ *                 "jacobian" 1 the two-wave kernel runs); the results are the
+ *                 same, the autotuner times both
  *   "jacobian"    0 (default): the Jacobian by difference quotients, one
  *                 right-hand side per column, as CVODE's FCVDENSE builds it for
  *                 the reference (finitechem.f90:1028); 1: the analytic Jacobian

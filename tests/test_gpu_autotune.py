@@ -29,7 +29,7 @@ def test_tune_and_apply_h2(tmp_path):
     R_d, S0, g, P = surrogate.block_states(fc, "h2", 16 ** 3, 5, 0)
     # the plain step's Jacobian and mapping change the rounding; the grid and
     # the batched right-hand side's mapping do not
-    for k in ("jacobian", "step_mapping"):
+    for k in ("jacobian", "step_mapping", "step_waves"):
         if k in rec:
             fc.set_option(k, rec[k])
     ref = S0.clone()

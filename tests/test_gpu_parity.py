@@ -302,3 +302,38 @@ def test_lane_mapping_step_vs_oracle(rtol, atol, oracle, torch_dev):
         assert np.sort(ewc.ravel())[-3] <= 1.0 and ewc.max() <= 10.0 and eT.max() < 1e-2
     if rtol < 1e-8:
         assert hr_rel < 1e-6
+
+
+def test_four_waves_per_simd_step_matches_two(oracle, torch_dev):
+    """The H2 wave-per-cell step kernel built for four resident waves per SIMD
+    (option step_waves 4: at most 128 VGPRs, two workgroups per CU) is the
+    same code under another register budget: on a ghosted, masked block it
+    gives the two-wave kernel's result and counters, and both stay within the
+    solver's error weight of the oracle."""
+    from tiger_nga_amd.finitechem import FiniteChem
+    nx, ny, nz = 8, 8, 8
+    g, RHO_, SC_, mask, P, dt = _grid_case("h2", "h2", nx, ny, nz, 2, 13, True)
+    res = {}
+    for waves in (4, 2):
+        fc = FiniteChem("h2", 1e-6, 1e-12)
+        fc.set_option("step_waves", waves)
+        S = SC_.copy(order="F")
+        HR = np.full(RHO_.shape, 3.0, order="F")
+        st = fc.source_chemistry(RHO_, S, g, dt, P, P, mask=mask, HR_gp=HR)
+        fc.close()
+        res[waves] = (S, HR, st)
+    S4, HR4, st4 = res[4]
+    S2, HR2, st2 = res[2]
+    print("\nh2 four waves per SIMD: steps %d (two waves %d), nfe %d, failed %d"
+          % (st4["nst"], st2["nst"], st4["nfe"], st4["failed"]))
+    assert st4["failed"] == 0 and st4["cells"] == st2["cells"]
+    for k in ("nst", "nfe", "nje", "nlu", "nni"):
+        assert st4[k] == st2[k], k
+    assert np.array_equal(S4, S2) and np.array_equal(HR4, HR2)
+    SC_o = SC_.copy(order="F")
+    oracle.source_chemistry("h2", RHO_, SC_o, g, dt, P, P, mask=mask, rtol=1e-6, atol=1e-12)
+    nsp = len(meta.species("h2"))
+    sp = slice(g.isc_sc - 1, g.isc_sc - 1 + nsp)
+    rho = RHO_[..., None]
+    ew = (np.abs(rho * (S4[..., sp] - SC_o[..., sp])) / (1e-6 * np.abs(rho * SC_o[..., sp]) + 1e-12)).max(axis=3)
+    assert np.sort(ew.ravel())[-3] <= 1.0 and ew.max() <= 10.0

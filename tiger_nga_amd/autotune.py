@@ -129,18 +129,25 @@ def tune(mech: str, cells: int = 32 ** 3, seed: int = 7, reps: int = 2, log=prin
     rec["grid"] = best_grid
     rec["measured"]["step_ms"] = {str(k): v * 1e3 for k, v in gt.items()}
     fc.set_option("grid", best_grid)
-    # the plain step's mapping: a wavefront per cell or a thread per cell
-    mt = {0: _time(torch, step, reps)}
+    # the plain step's mapping: a wavefront per cell (2 or, where supported, 4
+    # waves per SIMD) or a thread per cell
+    mt = {(0, 2): _time(torch, step, reps)}
     if fc.neq == 10:
+        fc.set_option("step_waves", 4)
+        mt[(0, 4)] = _time(torch, step, reps)
+        fc.set_option("step_waves", 2)
         fc.set_option("step_mapping", 1)
-        mt[1] = _time(torch, step, reps)
+        mt[(1, 2)] = _time(torch, step, reps)
         fc.set_option("step_mapping", 0)
+    names = {(0, 2): "wave", (0, 4): "wave, 4 per SIMD", (1, 2): "thread"}
     for k, v in mt.items():
-        log("  step mapping %s: %.2f ms" % ("thread per cell" if k else "wave per cell", v * 1e3))
-    best_step_map = min(mt, key=mt.get)
+        log("  step mapping %s: %.2f ms" % (names[k], v * 1e3))
+    best_step_map, best_waves = min(mt, key=mt.get)
     rec["step_mapping"] = best_step_map
-    rec["measured"]["step_mapping_ms"] = {("thread" if k else "wave"): v * 1e3 for k, v in mt.items()}
+    rec["step_waves"] = best_waves
+    rec["measured"]["step_mapping_ms"] = {names[k]: v * 1e3 for k, v in mt.items()}
     fc.set_option("step_mapping", best_step_map)
+    fc.set_option("step_waves", best_waves)
     # the plain step's Jacobian: difference quotients or analytic (the thread
     # per cell mapping has difference quotients only)
     jt = {}
@@ -148,7 +155,7 @@ def tune(mech: str, cells: int = 32 ** 3, seed: int = 7, reps: int = 2, log=prin
         fc.set_option("jacobian", jac)
         jt[jac] = _time(torch, step, reps)
         log("  step jacobian %s: %.2f ms" % ("analytic" if jac else "difference quotients", jt[jac] * 1e3))
-    best_jac = 1 if jt[1] < 0.98 * jt[0] and jt[1] < mt[best_step_map] else 0
+    best_jac = 1 if jt[1] < 0.98 * jt[0] and jt[1] < mt[(best_step_map, best_waves)] else 0
     rec["jacobian"] = best_jac
     rec["measured"]["step_jacobian_ms"] = {"difference_quotients": jt[0] * 1e3, "analytic": jt[1] * 1e3}
     fc.set_option("jacobian", best_jac)
@@ -270,7 +277,7 @@ def apply(fc, cells: int, path=CACHE) -> dict:
     done = {}
     if not rec:
         return done
-    for k in ("rhs_mapping", "grid", "step_mapping", "jacobian", "surr_adaptive", "surr_nsub", "surr_grade", "surr_max_att",
+    for k in ("rhs_mapping", "grid", "step_mapping", "step_waves", "jacobian", "surr_adaptive", "surr_nsub", "surr_grade", "surr_max_att",
               "surr_guess", "surr_h0_fac", "surr_jac_theta", "surr_max_jac", "surr_keep", "surr_tol_y"):
         if k in rec:
             fc.set_option(k, rec[k])

@@ -932,6 +932,7 @@ struct Bdf {
         return 0;
       }
       if (refresh && c_nje < max_jac) {
+        h = hh;  // the difference-quotient increments follow the step about to be taken
         if constexpr (AJ) {
           jac_analytic(yi);
         } else if (rc) {

@@ -26,7 +26,7 @@ __device__ __forceinline__ int wave_in_block() { return __builtin_amdgcn_readfir
 // wave pulls cells from a device counter until all are taken, so that cells
 // whose step counts differ by 10-100x keep every wave busy.
 template <int NEQ, bool AJ>
-__global__ __launch_bounds__(kBlock) void step_kernel(StepParams p) {
+__device__ __forceinline__ void step_body(const StepParams& p) {
   constexpr TabLayout L = tab_layout<NEQ>();
   __shared__ __attribute__((aligned(16))) char tab_s[L.bytes];
   __shared__ RhsLds<NEQ> rl[kWavesPerBlock];
@@ -60,6 +60,21 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepParams p) {
       }
     }
   }
+}
+
+// two waves per SIMD: up to 256 VGPRs per lane (every mechanism)
+template <int NEQ, bool AJ>
+__global__ __launch_bounds__(kBlock) void step_kernel(StepParams p) {
+  step_body<NEQ, AJ>(p);
+}
+
+// four waves per SIMD (two workgroups per CU): at most 128 VGPRs per lane, for
+// mechanisms whose tables and per-wave scratch leave room for a second
+// workgroup's copy in LDS (H2), with the difference-quotient Jacobian (the
+// analytic one needs more registers); the autotuner times it against the above
+template <int NEQ, bool AJ>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) void step_kernel_w4(StepParams p) {
+  step_body<NEQ, AJ>(p);
 }
 
 // ---------------------------------------------------------------------------
@@ -195,6 +210,18 @@ using namespace tgr;
 
 extern "C" int tgr_step_block_waves(void) { return kWavesPerBlock; }
 
+extern "C" int tgr_step_waves_supported(int neq, int waves) { return waves == 2 || (waves == 4 && neq == 10); }
+
+extern "C" int tgr_step_occupancy_w(int neq, int waves) {
+  if (waves == 4) {
+    int nb = 0;
+    hipError_t e = hipErrorInvalidValue;
+    if (neq == 10) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, step_kernel_w4<10, false>, kBlock, 0);
+    return e == hipSuccess ? nb : 0;
+  }
+  return tgr_step_occupancy(neq);
+}
+
 extern "C" int tgr_step_occupancy(int neq) {
   int nb = 0;
   hipError_t e = hipErrorInvalidValue;
@@ -213,8 +240,13 @@ extern "C" int tgr_launch_step(const StepParams* p, int neq, int grid, hipStream
   hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, p->queue, p->totals);
   switch (neq) {
     case 10:
-      if (p->jac) hipLaunchKernelGGL((step_kernel<10, true>), dim3(grid), dim3(kBlock), 0, s, *p);
-      else hipLaunchKernelGGL((step_kernel<10, false>), dim3(grid), dim3(kBlock), 0, s, *p);
+      if (p->waves == 4 && !p->jac) {
+        hipLaunchKernelGGL((step_kernel_w4<10, false>), dim3(grid), dim3(kBlock), 0, s, *p);
+      } else if (p->jac) {
+        hipLaunchKernelGGL((step_kernel<10, true>), dim3(grid), dim3(kBlock), 0, s, *p);
+      } else {
+        hipLaunchKernelGGL((step_kernel<10, false>), dim3(grid), dim3(kBlock), 0, s, *p);
+      }
       break;
     case 37:
       if (p->jac) hipLaunchKernelGGL((step_kernel<37, true>), dim3(grid), dim3(kBlock), 0, s, *p);

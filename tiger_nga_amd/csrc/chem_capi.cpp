@@ -76,6 +76,7 @@ struct tgr_chem {
   int max_steps = 10000, qmax = 5;
   int jac = 0;  // 0: difference-quotient Jacobian (the reference's CVODE FCVDENSE); 1: analytic
   int step_mapping = 0;  // 0: a wave per cell; 1: a thread per cell (small mechanisms, bdf_lane.h)
+  int step_waves = 2;    // wave per cell: resident waves per SIMD (2, or 4 where the kernel supports it)
   int rhs_lanes = 1;  // batched right-hand side: one state per lane (1) or per wave (0)
   int64_t grid_override = 0;
   int device = 0;
@@ -176,6 +177,10 @@ int check_grid(const tgr_grid* g, int nsp) {
 }
 
 // step parameters of a block (no queue / workspace yet)
+// waves per SIMD of the wave-per-cell step kernel in force: the four-wave
+// build has the difference-quotient Jacobian only
+int step_waves_eff(const tgr_chem* c) { return c->jac ? 2 : c->step_waves; }
+
 StepParams step_params(const tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const tgr_grid* g,
                        double* rho, double* sc, int isc_sc0, int isc_T0, const int* mask, double* hr, double* chem) {
   const int nx = g->imax_ - g->imin_ + 1, ny = g->jmax_ - g->jmin_ + 1, nz = g->kmax_ - g->kmin_ + 1;
@@ -189,6 +194,7 @@ StepParams step_params(const tgr_chem* c, double dt, double p_thermo, double p_t
   p.max_steps = c->max_steps;
   p.qmax = c->qmax;
   p.jac = c->jac;
+  p.waves = step_waves_eff(c);
   p.nx = nx;
   p.ny = ny;
   p.nz = nz;
@@ -264,7 +270,8 @@ int run_step(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const
                                c->grid_override > 0 ? c->grid_override
                                                     : (int64_t)c->n_cu * std::max(1, tgr_step_lane_occupancy(c->neq)),
                                (p.ncell + 63) / 64))
-                         : persistent_grid(c, tgr_step_occupancy(c->neq), p.ncell, tgr_step_block_waves());
+                         : persistent_grid(c, tgr_step_occupancy_w(c->neq, step_waves_eff(c)), p.ncell,
+                                           tgr_step_block_waves());
   const int wpb = tgr_step_block_waves();
   const size_t ws_bytes = lanes ? (size_t)grid * tgr_step_lane_ws(c->neq) * sizeof(double)
                                 : (size_t)grid * wpb * c->neq * 64 * sizeof(double);
@@ -413,6 +420,10 @@ int tgr_chem_set_option(tgr_chem* c, const char* key, int64_t v) {
   else if (!strcmp(key, "rhs_mapping")) {
     if (v != 0 && v != 1) return fail(TGR_E_ARG, "rhs_mapping must be 0 (wave per state) or 1 (lane per state)");
     c->rhs_lanes = (int)v;
+  } else if (!strcmp(key, "step_waves")) {
+    if (!tgr_step_waves_supported(c->neq, (int)v))
+      return fail(TGR_E_ARG, "step_waves must be 2, or 4 for a 10-equation mechanism (H2)");
+    c->step_waves = (int)v;
   } else if (!strcmp(key, "step_mapping")) {
     if (v != 0 && v != 1) return fail(TGR_E_ARG, "step_mapping must be 0 (wave per cell) or 1 (thread per cell)");
     c->step_mapping = (int)v;
@@ -628,7 +639,8 @@ int run_surrogate(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, 
     hsub[n - 1] += dt - acc;  // exact sum
   }
   const int grid_s = persistent_grid(c, tgr_surr_occupancy(c->neq, adaptive), p.ncell, tgr_surr_block_waves());
-  const int grid_p = persistent_grid(c, tgr_step_occupancy(c->neq), p.ncell, tgr_step_block_waves());
+  const int grid_p =
+      persistent_grid(c, tgr_step_occupancy_w(c->neq, step_waves_eff(c)), p.ncell, tgr_step_block_waves());
   // the workspace serves both kernels: each wave of each block owns neq x 64 doubles
   const size_t ws_waves = std::max((size_t)grid_s * tgr_surr_block_waves(), (size_t)grid_p * tgr_step_block_waves());
   // every buffer sized before the first launch (a resize frees memory)

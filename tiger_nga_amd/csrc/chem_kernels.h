@@ -9,6 +9,8 @@
 
 extern "C" {
 int tgr_step_occupancy(int neq);
+int tgr_step_occupancy_w(int neq, int waves);
+int tgr_step_waves_supported(int neq, int waves);
 int tgr_step_block_waves(void);
 int tgr_launch_step(const tgr::StepParams* p, int neq, int grid, hipStream_t s);
 int tgr_launch_prodrates(const tgr::DevMech* m, int64_t n, const double* C, const double* T, const double* P,

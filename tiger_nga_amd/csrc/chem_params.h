@@ -28,6 +28,7 @@ struct StepParams {
   double dt, p_gp, rtol, atol;
   int max_steps, qmax;
   int jac;              // 0: difference-quotient Jacobian (CVODE's, as the reference); 1: analytic
+  int waves;            // wave-per-cell step kernel: resident waves per SIMD (2; 4 for 10 equations)
   int nx, ny, nz;
   int64_t sx, sy, sz, base, plane;
   int isc_sc0, isc_T0;  // 0-based scalar planes of the first species and of T

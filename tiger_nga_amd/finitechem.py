@@ -232,9 +232,9 @@ class FiniteChem:
             pass
 
     def set_option(self, key: str, value):
-        """Integer knobs ("grid", "rhs_mapping", "jacobian", "max_order", "max_steps", "surr_nsub",
-        "surr_guess", "surr_newton_max", "surr_probe", "surr_adaptive", "surr_max_att", "surr_max_jac") or real
-        ones (REAL_OPTIONS)."""
+        """Integer knobs ("grid", "rhs_mapping", "step_mapping", "step_waves", "jacobian", "max_order", "max_steps",
+        "surr_nsub", "surr_guess", "surr_newton_max", "surr_probe", "surr_adaptive", "surr_max_att", "surr_max_jac",
+        "surr_mode", "surr_split_jac") or real ones (REAL_OPTIONS)."""
         if key in REAL_OPTIONS:
             _check(self.lib, self.lib.tgr_chem_set_option_real(self._h, key.encode(), float(value)), "set_option")
         else:

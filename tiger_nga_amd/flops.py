@@ -70,6 +70,14 @@ def step_bytes(info: dict, cells: int) -> float:
     return 8.0 * cells * ((nsp + 2) + (nsp + 1) + 4)
 
 
+def diffusion_bytes(nsp: int, cells: int) -> float:
+    """Algorithmic HBM bytes of one finitechem_diffusion call on the interior
+    cells, 8 bytes each: read rho, Y_k and T, and the diffusivities of the
+    species and T; write diff_corr (species and T) and the three diff_gp
+    planes.  Stencil neighbours and intermediate planes are not counted."""
+    return 8.0 * cells * (1 + 3 * (nsp + 1) + 3)
+
+
 def properties_flops(nsp: int) -> float:
     """Algorithmic flops of one cell of the properties kernel (same convention;
     sqrt = 1): W_mix (3 nsp), species viscosities (omegamu 16 + 3 per species),
