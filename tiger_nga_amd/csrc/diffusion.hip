@@ -188,6 +188,31 @@ __device__ __forceinline__ Flux face_flux(const DiffParams& p, int d, int i, int
   return f;
 }
 
+// Gas radiation of the temperature equation (finitechem.f90:738-792):
+// optically thin loss of H2O, CO2, CH4 and CO with the reference's Planck-mean
+// absorption fits, Tb = 298 K; a species the mechanism lacks contributes 0.
+__device__ __forceinline__ double gas_radiation(const DiffParams& p, double T, int64_t o) {
+  const DiffGeom& g = p.gm;
+  const double tm1 = 1000.0 / T;
+  const double T2 = T * T;
+  const double aH2O =
+      -0.23093 + tm1 * (-1.12390 + tm1 * (9.41530 + tm1 * (-2.99880 + tm1 * (0.51382 + tm1 * -1.86840E-05))));
+  const double aCO2 = 18.741 + tm1 * (-121.310 + tm1 * (273.500 + tm1 * (-194.050 + tm1 * (56.310 + tm1 * -5.8169))));
+  const double aCH4 = 6.6334 - 0.0035686 * T + 1.6682e-08 * T2 + 2.5611e-10 * T2 * T - 2.6558e-14 * T2 * T2;
+  const bool lo = T <= 750.0;
+  const double aCO = lo ? 4.7869 + T * (-0.06953 + T * (2.95775e-4 + T * (-4.25732e-7 + T * 2.02894e-10)))
+                        : 10.09 + T * (-0.01183 + T * (4.7753e-6 + T * (-5.87209e-10 + T * -2.5334e-14)));
+  const double* S = p.sc;
+  double sum = 0.0;
+  if (p.i_co2 >= 0) sum += aCO2 * S[(int64_t)(g.isc_sc0 + p.i_co2) * g.plane + o] / p.w_co2;
+  if (p.i_h2o >= 0) sum += aH2O * S[(int64_t)(g.isc_sc0 + p.i_h2o) * g.plane + o] / p.w_h2o;
+  if (p.i_ch4 >= 0) sum += aCH4 * S[(int64_t)(g.isc_sc0 + p.i_ch4) * g.plane + o] / p.w_ch4;
+  if (p.i_co >= 0) sum += aCO * S[(int64_t)(g.isc_sc0 + p.i_co) * g.plane + o] / p.w_co;
+  const double radconst = 4.0 * 5.67051e-8 * 8314.34 / 101320.0;
+  const double T4 = T2 * T2, Tb = 298.0;
+  return -radconst * p.rho[o] * T * (T4 - Tb * Tb * Tb * Tb) * sum;
+}
+
 }  // namespace
 
 template <int NEQ>
@@ -249,27 +274,7 @@ __global__ __launch_bounds__(256) void diff_main_kernel(DiffParams p) {
         if (slot >= 0) p.dgp[(int64_t)slot * g.plane + o] = dfick + dcor;
       }
     }
-    if (p.gasrad) {
-      const double tm1 = 1000.0 / T;
-      const double T2 = T * T;
-      const double aH2O =
-          -0.23093 + tm1 * (-1.12390 + tm1 * (9.41530 + tm1 * (-2.99880 + tm1 * (0.51382 + tm1 * -1.86840E-05))));
-      const double aCO2 =
-          18.741 + tm1 * (-121.310 + tm1 * (273.500 + tm1 * (-194.050 + tm1 * (56.310 + tm1 * -5.8169))));
-      const double aCH4 = 6.6334 - 0.0035686 * T + 1.6682e-08 * T2 + 2.5611e-10 * T2 * T - 2.6558e-14 * T2 * T2;
-      const bool lo = T <= 750.0;
-      const double aCO = lo ? 4.7869 + T * (-0.06953 + T * (2.95775e-4 + T * (-4.25732e-7 + T * 2.02894e-10)))
-                            : 10.09 + T * (-0.01183 + T * (4.7753e-6 + T * (-5.87209e-10 + T * -2.5334e-14)));
-      const double* S = p.sc;
-      double sum = 0.0;
-      if (p.i_co2 >= 0) sum += aCO2 * S[(int64_t)(g.isc_sc0 + p.i_co2) * g.plane + o] / p.w_co2;
-      if (p.i_h2o >= 0) sum += aH2O * S[(int64_t)(g.isc_sc0 + p.i_h2o) * g.plane + o] / p.w_h2o;
-      if (p.i_ch4 >= 0) sum += aCH4 * S[(int64_t)(g.isc_sc0 + p.i_ch4) * g.plane + o] / p.w_ch4;
-      if (p.i_co >= 0) sum += aCO * S[(int64_t)(g.isc_sc0 + p.i_co) * g.plane + o] / p.w_co;
-      const double radconst = 4.0 * 5.67051e-8 * 8314.34 / 101320.0;
-      const double T4 = T2 * T2, Tb = 298.0;
-      eT += -radconst * p.rho[o] * T * (T4 - Tb * Tb * Tb * Tb) * sum;
-    }
+    if (p.gasrad) eT += gas_radiation(p, T, o);
     const double dT = p.diff[(int64_t)g.isc_T0 * g.plane + o];
     eT = eT + dT * cx * tx;
     eT = eT + dT * cy * ty;
@@ -277,6 +282,231 @@ __global__ __launch_bounds__(256) void diff_main_kernel(DiffParams p) {
     eT = eT / p.cpm[o] * p.dt;
     if (p.corr) p.corr[(int64_t)N * g.plane + o] = eT;
     if (p.src) p.src[(int64_t)g.isc_T0 * g.plane + o] += eT;
+  }
+}
+
+// ---- second order (st1 = 0, st2 = 1) ----------------------------------------
+// The same sums as the kernels above, in the same order, with what does not
+// depend on the species hoisted out of the species loop: the operator
+// coefficients, the molar factors w / W_gp and sum grad * W_gp, the correction
+// velocity at each face; per species only Y and D at the cell and its six
+// neighbours are loaded (each once), instead of once per face and stencil
+// point.
+
+// correction velocities at the face cells (diff_face_kernel, st1 = 0, st2 = 1:
+// a face interpolates its cell and the one before it in each direction)
+template <int NEQ>
+__global__ __launch_bounds__(256) void diff_face2_kernel(DiffParams p) {
+  constexpr int N = NEQ - 1;
+  const DiffGeom& g = p.gm;
+  const tgr_metric_dev& m = p.m;
+  const int fi0 = g.imin_, fni = g.imax_ + 1 - fi0 + 1;
+  const int fj0 = g.jmin_, fnj = g.jmax_ + 1 - fj0 + 1;
+  const int fk0 = g.kmin_, fnk = g.kmax_ + 1 - fk0 + 1;
+  const int64_t n = (int64_t)fni * fnj * fnk;
+  const int64_t sy = g.nxo, sz = (int64_t)g.nxo * g.nyo;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+    const int i = fi0 + (int)(q % fni), j = fj0 + (int)((q / fni) % fnj), k = fk0 + (int)(q / ((int64_t)fni * fnj));
+    const int64_t o = c3(g, i, j, k);
+    double vx = 0.0, vy = 0.0, vz = 0.0;
+    if (!wall(g, p.mask, i, j)) {
+      // coefficients of the two stencil points (s = -1, 0) per direction
+      double w[3][2], gg[3][2], wi[3][2], gw[3];
+      const int64_t nb[3] = {o - 1, o - sy, o - sz};
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int s = t - 1;
+        w[0][t] = op(m.interp_sc_x, i, fi0, fni, j, g.jmino_, g.nyo, s, -1);
+        w[1][t] = op(m.interp_sc_y, i, g.imino_, g.nxo, j, fj0, fnj, s, -1);
+        w[2][t] = op(m.interp_sc_z, i, g.imino_, g.nxo, j, fj0, fnj, s, -1);
+        gg[0][t] = op(m.grad_x, i, fi0, fni, j, fj0, fnj, s, -1);
+        gg[1][t] = op(m.grad_y, i, fi0, fni, j, fj0, fnj, s, -1);
+        gg[2][t] = op(m.grad_z, i, fi0, fni, j, fj0, fnj, s, -1);
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        gw[d] = 0.0;
+        if (p.molar) {
+          const double w0 = p.wg[nb[d]], w1 = p.wg[o];
+          gw[d] += gg[d][0] * w0;
+          gw[d] += gg[d][1] * w1;
+          wi[d][0] = 1.0 / w0;
+          wi[d][1] = 1.0 / w1;
+        }
+      }
+      double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll 2
+      for (int sp = 0; sp < N; ++sp) {
+        const double* Y = p.sc + (int64_t)(g.isc_sc0 + sp) * g.plane;
+        const double* D = p.diff + (int64_t)(g.isc_sc0 + sp) * g.plane;
+        const double yc = Y[o], dc = D[o];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double y0 = Y[nb[d]], d0 = D[nb[d]];
+          double a = 0, b = 0, ii = 0;
+          a += w[d][0] * d0;
+          b += gg[d][0] * y0;
+          a += w[d][1] * dc;
+          b += gg[d][1] * yc;
+          acc[d] += a * b;
+          if (p.molar) {
+            ii += w[d][0] * (y0 * wi[d][0]);
+            ii += w[d][1] * (yc * wi[d][1]);
+            acc[d] += a * ii * gw[d];
+          }
+        }
+      }
+      vx = acc[0];
+      vy = acc[1];
+      vz = acc[2];
+    }
+    p.cvel[o] = vx;
+    p.cvel[g.plane + o] = vy;
+    p.cvel[2 * g.plane + o] = vz;
+  }
+}
+
+// the per-cell terms (diff_main_kernel, st1 = 0, st2 = 1: faces at the cell
+// and the next one in each direction, each interpolating two cells).  The
+// species loop reads the operator coefficients through restrict pointers, so
+// that the compiler keeps the loop-invariant ones in registers as far as they
+// fit; per species it loads Y and D at the cell and its six neighbours once.
+template <int NEQ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void diff_main2_kernel(DiffParams p) {
+  constexpr TabLayout L = tab_layout<NEQ>();
+  constexpr int N = NEQ - 1;
+  const DiffGeom& g = p.gm;
+  const int nx = g.imax_ - g.imin_ + 1, ny = g.jmax_ - g.jmin_ + 1, nz = g.kmax_ - g.kmin_ + 1;
+  const int fi0 = g.imin_, fni = nx + 1, fj0 = g.jmin_, fnj = ny + 1;
+  const int64_t n = (int64_t)nx * ny * nz;
+  const int64_t sy = g.nxo, sz = (int64_t)g.nxo * g.nyo;
+  const double* __restrict__ isx = p.m.interp_sc_x;
+  const double* __restrict__ isy = p.m.interp_sc_y;
+  const double* __restrict__ isz = p.m.interp_sc_z;
+  const double* __restrict__ grx = p.m.grad_x;
+  const double* __restrict__ gry = p.m.grad_y;
+  const double* __restrict__ grz = p.m.grad_z;
+  const double* __restrict__ dvu = p.m.div_u;
+  const double* __restrict__ dvv = p.m.div_v;
+  const double* __restrict__ dvw = p.m.div_w;
+  const double* __restrict__ imx = p.m.interp_u_xm;
+  const double* __restrict__ imy = p.m.interp_v_ym;
+  const double* __restrict__ imz = p.m.interp_w_zm;
+  const double* __restrict__ sc = p.sc;
+  const double* __restrict__ df = p.diff;
+  const double* __restrict__ wgp = p.wg;
+  const double* __restrict__ cpm = p.cpm;
+  const double* __restrict__ cvel = p.cvel;
+  const char* __restrict__ tab = p.mech.tab;
+  double* __restrict__ corr = p.corr;
+  double* __restrict__ src = p.src;
+  double* __restrict__ dgp = p.dgp;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+    const int i = g.imin_ + (int)(q % nx), j = g.jmin_ + (int)((q / nx) % ny), k = g.kmin_ + (int)(q / ((int64_t)nx * ny));
+    const int64_t o = c3(g, i, j, k);
+    if (wall(g, p.mask, i, j)) continue;
+    const double* Tf = sc + (int64_t)g.isc_T0 * g.plane;
+    const double T = Tf[o];
+    // cell gradients of T and cp_mix (stp = 1)
+    double tx = 0, ty = 0, tz = 0, cx = 0, cy = 0, cz = 0;
+#pragma unroll
+    for (int s = -1; s <= 1; ++s) {
+      const double gxm = op(p.m.grad_xm, i, g.imin_, nx, j, g.jmin_, ny, s, -1);
+      const double gym = op(p.m.grad_ym, i, g.imin_, nx, j, g.jmin_, ny, s, -1);
+      const double gzm = op(p.m.grad_zm, i, g.imin_, nx, j, g.jmin_, ny, s, -1);
+      tx += gxm * Tf[o + s];
+      ty += gym * Tf[o + s * sy];
+      tz += gzm * Tf[o + s * sz];
+      cx += gxm * cpm[o + s];
+      cy += gym * cpm[o + s * sy];
+      cz += gzm * cpm[o + s * sz];
+    }
+    const int64_t step[3] = {1, sy, sz};
+    // wall faces carry no flux (the z faces of a non-wall cell are never walls)
+    const bool live[3][2] = {{true, !wall(g, p.mask, i + 1, j)}, {true, !wall(g, p.mask, i, j + 1)}, {true, true}};
+    double eT = 0.0;  // temperature term before the cp-gradient part and / cp
+#pragma unroll 1
+    for (int sp = 0; sp < N; ++sp) {
+      const double* Y = sc + (int64_t)(g.isc_sc0 + sp) * g.plane;
+      const double* D = df + (int64_t)(g.isc_sc0 + sp) * g.plane;
+      const double yc = Y[o], dc = D[o];
+      double dcor = 0.0, dfick = 0.0, fm[3] = {0.0, 0.0, 0.0};
+      double fc[3][2], ff[3][2];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double ym = Y[o - step[d]], yp = Y[o + step[d]], dm = D[o - step[d]], dp = D[o + step[d]];
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+          // the face cell is (i, j, k) + f along d; its two points are
+          // (-1, 0) for f = 0 and (0, +1) for f = 1
+          const int fi = d == 0 ? i + f : i, fj = d == 1 ? j + f : j;
+          const int64_t fo = o + f * step[d];
+          const double y0 = f ? yc : ym, y1 = f ? yp : yc;
+          const double d0 = f ? dc : dm, d1 = f ? dp : dc;
+          double w0, w1, g0, g1;
+          if (d == 0) {
+            w0 = op(isx, fi, fi0, fni, fj, g.jmino_, g.nyo, -1, -1);
+            w1 = op(isx, fi, fi0, fni, fj, g.jmino_, g.nyo, 0, -1);
+          } else {
+            const double* ws = d == 1 ? isy : isz;
+            w0 = op(ws, fi, g.imino_, g.nxo, fj, fj0, fnj, -1, -1);
+            w1 = op(ws, fi, g.imino_, g.nxo, fj, fj0, fnj, 0, -1);
+          }
+          const double* gs = d == 0 ? grx : (d == 1 ? gry : grz);
+          g0 = op(gs, fi, fi0, fni, fj, fj0, fnj, -1, -1);
+          g1 = op(gs, fi, fi0, fni, fj, fj0, fnj, 0, -1);
+          double syf = 0, a = 0, b = 0;
+          syf += w0 * y0;
+          a += w0 * d0;
+          b += g0 * y0;
+          syf += w1 * y1;
+          a += w1 * d1;
+          b += g1 * y1;
+          double c = -cvel[(int64_t)d * g.plane + fo] * syf;
+          if (p.molar) {
+            const double wa = wgp[fo - step[d]], wb = wgp[fo];
+            double iyw = 0, gw = 0;
+            iyw += w0 * (y0 / wa);
+            gw += g0 * wa;
+            iyw += w1 * (y1 / wb);
+            gw += g1 * wb;
+            c += a * iyw * gw;
+          }
+          fc[d][f] = live[d][f] ? c : 0.0;
+          ff[d][f] = live[d][f] ? a * b : 0.0;
+        }
+      }
+#pragma unroll
+      for (int f = 0; f < 2; ++f) {
+        const double du = op(dvu, i, g.imin_, nx, j, g.jmin_, ny, f, 0);
+        const double dv = op(dvv, i, g.imin_, nx, j, g.jmin_, ny, f, 0);
+        const double dw = op(dvw, i, g.imin_, nx, j, g.jmin_, ny, f, 0);
+        dcor += du * fc[0][f] + dv * fc[1][f] + dw * fc[2][f];
+        dfick += du * ff[0][f] + dv * ff[1][f] + dw * ff[2][f];
+        fm[0] += op(imx, i, g.imin_, nx, j, g.jmin_, ny, f, 0) * (fc[0][f] + ff[0][f]);
+        fm[1] += op(imy, i, g.imin_, nx, j, g.jmin_, ny, f, 0) * (fc[1][f] + ff[1][f]);
+        fm[2] += op(imz, i, g.imin_, nx, j, g.jmin_, ny, f, 0) * (fc[2][f] + ff[2][f]);
+      }
+      double cpk;
+      cp_species(tab, L, p.mech.tmid, T, sp, cpk);
+      eT = eT + cpk * fm[0] * tx;
+      eT = eT + cpk * fm[1] * ty;
+      eT = eT + cpk * fm[2] * tz;
+      if (corr) corr[(int64_t)sp * g.plane + o] = dcor * p.dt;
+      if (src) src[(int64_t)(g.isc_sc0 + sp) * g.plane + o] += dcor * p.dt;
+      if (dgp) {
+        const int slot = sp == p.i_h2o ? 0 : (sp == p.i_co ? 1 : (sp == p.i_co2 ? 2 : -1));
+        if (slot >= 0) dgp[(int64_t)slot * g.plane + o] = dfick + dcor;
+      }
+    }
+    if (p.gasrad) eT += gas_radiation(p, T, o);
+    const double dT = df[(int64_t)g.isc_T0 * g.plane + o];
+    eT = eT + dT * cx * tx;
+    eT = eT + dT * cy * ty;
+    eT = eT + dT * cz * tz;
+    eT = eT / cpm[o] * p.dt;
+    if (corr) corr[(int64_t)N * g.plane + o] = eT;
+    if (src) src[(int64_t)g.isc_T0 * g.plane + o] += eT;
   }
 }
 
@@ -292,10 +522,18 @@ extern "C" int tgr_launch_diffusion(const DiffParams* p, int neq, hipStream_t s)
                         (g.kmax_ - g.kmin_ + 1 + st1 + st2);
   const int64_t nint = (int64_t)(g.imax_ - g.imin_ + 1) * (g.jmax_ - g.jmin_ + 1) * (g.kmax_ - g.kmin_ + 1);
   auto blocks = [](int64_t n) { return (unsigned)std::min<int64_t>((n + 255) / 256, 65535); };
+  // second order (the reference's default stencil) has its own pair of
+  // kernels; other orders take the general ones
+  const bool o2 = st1 == 0 && st2 == 1;
 #define TGR_DIFF_LAUNCH(NEQ_)                                                                  \
   hipLaunchKernelGGL(diff_cell_kernel<NEQ_>, dim3(blocks(ncell)), dim3(256), 0, s, *p);       \
-  hipLaunchKernelGGL(diff_face_kernel<NEQ_>, dim3(blocks(nface)), dim3(256), 0, s, *p);       \
-  hipLaunchKernelGGL(diff_main_kernel<NEQ_>, dim3(blocks(nint)), dim3(256), 0, s, *p);
+  if (o2) {                                                                                     \
+    hipLaunchKernelGGL(diff_face2_kernel<NEQ_>, dim3(blocks(nface)), dim3(256), 0, s, *p);    \
+    hipLaunchKernelGGL(diff_main2_kernel<NEQ_>, dim3(blocks(nint)), dim3(256), 0, s, *p);     \
+  } else {                                                                                      \
+    hipLaunchKernelGGL(diff_face_kernel<NEQ_>, dim3(blocks(nface)), dim3(256), 0, s, *p);     \
+    hipLaunchKernelGGL(diff_main_kernel<NEQ_>, dim3(blocks(nint)), dim3(256), 0, s, *p);      \
+  }
   switch (neq) {
     case 10: TGR_DIFF_LAUNCH(10) break;
     case 37: TGR_DIFF_LAUNCH(37) break;
