@@ -24,6 +24,18 @@ module tiger_chem
      integer(c_int) :: nscalar, isc_sc, isc_T
   end type tgr_grid
 
+  ! metric operators of the flow solver (device arrays), tgr_metric in the C
+  ! header; TGR_DIFF_* are the flags of tgr_chem_diffusion_device
+  type, bind(C), public :: tgr_metric
+     integer(c_int) :: st1, st2
+     type(c_ptr) :: interp_sc_x, interp_sc_y, interp_sc_z
+     type(c_ptr) :: grad_x, grad_y, grad_z
+     type(c_ptr) :: div_u, div_v, div_w
+     type(c_ptr) :: interp_u_xm, interp_v_ym, interp_w_zm
+     type(c_ptr) :: grad_xm, grad_ym, grad_zm
+  end type tgr_metric
+  integer, parameter, public :: TGR_DIFF_MOLAR = 1, TGR_DIFF_GASRAD = 2, TGR_DIFF_CV = 4
+
   ! per-call integrator statistics, tgr_stats in the C header
   type, bind(C), public :: tgr_stats
      integer(c_int64_t) :: cells, nst, nfe, nfe_dq, nje, nsetups, nni, netf, ncfn, nlu, failed
@@ -33,7 +45,7 @@ module tiger_chem
   public :: tgr_chem_create, tgr_chem_destroy, tgr_chem_source_chemistry, tgr_chem_source_chemistry_device
   public :: tgr_chem_molar_mass, tgr_chem_species_name, tgr_chem_set_option, tgr_chem_set_option_real
   public :: tgr_chem_last_error
-  public :: tgr_chem_properties_device, tgr_chem_temperature_device
+  public :: tgr_chem_properties_device, tgr_chem_temperature_device, tgr_chem_diffusion_device
   public :: tgr_chem_load_surrogate, tgr_chem_source_chemistry_surrogate
   public :: tiger_chem_load_surrogate, tiger_chem_source_chemistry_surrogate
   public :: tiger_chem_init, tiger_chem_final, tiger_chem_source_chemistry, tiger_chem_message
@@ -112,6 +124,20 @@ module tiger_chem
        integer(c_int), value :: isc_zmix
        type(c_ptr), value :: rho, visc, diff, stream
      end function tgr_chem_properties_device
+
+     ! finitechem_diffusion (and with src the update of
+     ! finitechem_source_transport) on device arrays; diff_corr, diff_gp,
+     ! src and mask may be c_null_ptr
+     integer(c_int) function tgr_chem_diffusion_device(ctx, dt, g, m, flags, rho, sc, diff, mask, diff_corr, &
+          diff_gp, src, stream) bind(C, name="tgr_chem_diffusion_device")
+       import :: c_int, c_ptr, c_double, tgr_grid, tgr_metric
+       type(c_ptr), value :: ctx
+       real(c_double), value :: dt
+       type(tgr_grid), intent(in) :: g
+       type(tgr_metric), intent(in) :: m
+       integer(c_int), value :: flags
+       type(c_ptr), value :: rho, sc, diff, mask, diff_corr, diff_gp, src, stream
+     end function tgr_chem_diffusion_device
 
      ! temperature from mixture enthalpy, device arrays
      integer(c_int) function tgr_chem_temperature_device(ctx, n, Y, hmix, T0, T, iters, stream) &
