@@ -207,3 +207,27 @@ def test_standalone_surrogate(torch_dev, mech, workload, relax):
                  ss["rej_newton"], ss["rej_error"], ss["rej_enthalpy"]))
         assert err <= 4e-6 and cons <= 1e-12
     fc.close()
+
+
+def test_fallback_through_thread_per_cell_step(torch_dev):
+    """The guided call with the plain step in its thread-per-cell mapping
+    (step_mapping 1, H2): the rejected cells go through that kernel, whose
+    workspace the call sizes before its first launch; every cell ends either
+    accepted or bit-identical to the thread-per-cell plain step."""
+    torch = torch_dev
+    from tiger_nga_amd import surrogate
+    from tiger_nga_amd.finitechem import FiniteChem
+    fc = FiniteChem("h2", 1e-6, 1e-12)
+    fc.set_option("step_mapping", 1)
+    R_d, S0, g, P = surrogate.block_states(fc, "h2", 16 ** 3, 3, 0)
+    Sp = S0.clone()
+    stp = fc.source_chemistry(R_d, Sp, g, 1e-6, P, P)
+    _model("h2").attach(fc)
+    fc.set_option("surr_probe", 0)
+    Ss = S0.clone()
+    st, ss = fc.source_chemistry_surrogate(R_d, Ss, g, 1e-6, P, P)
+    fc.close()
+    print("\nh2 guided with the thread-per-cell fallback: accepted %d, rejected %d" % (ss["accepted"], ss["rejected"]))
+    assert stp["failed"] == 0 and st["failed"] == 0 and ss["accepted"] + ss["rejected"] == 16 ** 3
+    same = (Ss == Sp).all(dim=3)
+    assert ss["rejected"] > 0 and int((~same).sum()) <= ss["accepted"]

@@ -244,6 +244,24 @@ int persistent_grid(const tgr_chem* c, int occupancy, int64_t ncell, int wpb) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(gs, (ncell + wpb - 1) / wpb));
 }
 
+// Launch shape of the plain step over ncell cells: the grid, whether the
+// thread-per-cell kernel runs, and the solver workspace it needs (bytes).  A
+// thread per cell: one 64-thread workgroup per wave slot, each thread taking
+// cells one by one (persistent grid of the resident capacity); a wave per
+// cell: the persistent grid of the kernel's occupancy, neq x 64 doubles per wave.
+size_t step_shape(const tgr_chem* c, int64_t ncell, int* grid, bool* lanes) {
+  *lanes = c->step_mapping == 1 && tgr_step_lane_supported(c->neq) && !c->jac;
+  if (*lanes) {
+    *grid = (int)std::max<int64_t>(
+        1, std::min<int64_t>(c->grid_override > 0 ? c->grid_override
+                                                   : (int64_t)c->n_cu * std::max(1, tgr_step_lane_occupancy(c->neq)),
+                             (ncell + 63) / 64));
+    return (size_t)*grid * tgr_step_lane_ws(c->neq) * sizeof(double);
+  }
+  *grid = persistent_grid(c, tgr_step_occupancy_w(c->neq, step_waves_eff(c)), ncell, tgr_step_block_waves());
+  return (size_t)*grid * tgr_step_block_waves() * c->neq * 64 * sizeof(double);
+}
+
 // Layout of an external scratch block (tgr_chem_source_chemistry_device_noalloc):
 // the work counter at byte 0, the counter totals at byte 64, the per-wave
 // Jacobian workspace from byte 256.
@@ -263,18 +281,9 @@ int run_step(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, const
   StepParams p = step_params(c, dt, p_thermo, p_thermo_old, g, rho, sc, isc_sc0, isc_T0, mask, hr, chem);
   p.cell_list = list;
   p.nlist = nlist;
-  // a thread per cell: one 64-thread workgroup per wave slot, each thread
-  // takes cells one by one (persistent grid of the resident capacity)
-  const bool lanes = c->step_mapping == 1 && tgr_step_lane_supported(c->neq) && !c->jac;
-  const int grid = lanes ? (int)std::max<int64_t>(1, std::min<int64_t>(
-                               c->grid_override > 0 ? c->grid_override
-                                                    : (int64_t)c->n_cu * std::max(1, tgr_step_lane_occupancy(c->neq)),
-                               (p.ncell + 63) / 64))
-                         : persistent_grid(c, tgr_step_occupancy_w(c->neq, step_waves_eff(c)), p.ncell,
-                                           tgr_step_block_waves());
-  const int wpb = tgr_step_block_waves();
-  const size_t ws_bytes = lanes ? (size_t)grid * tgr_step_lane_ws(c->neq) * sizeof(double)
-                                : (size_t)grid * wpb * c->neq * 64 * sizeof(double);
+  bool lanes = false;
+  int grid = 0;
+  const size_t ws_bytes = step_shape(c, p.ncell, &grid, &lanes);
   if (scratch) {
     if (ws_bytes > scratch_bytes - kScrWs)
       return fail(TGR_E_ARG, "the step's workspace (" + std::to_string(ws_bytes) + " bytes for a grid of " +
@@ -639,12 +648,14 @@ int run_surrogate(tgr_chem* c, double dt, double p_thermo, double p_thermo_old, 
     hsub[n - 1] += dt - acc;  // exact sum
   }
   const int grid_s = persistent_grid(c, tgr_surr_occupancy(c->neq, adaptive), p.ncell, tgr_surr_block_waves());
-  const int grid_p =
-      persistent_grid(c, tgr_step_occupancy_w(c->neq, step_waves_eff(c)), p.ncell, tgr_step_block_waves());
-  // the workspace serves both kernels: each wave of each block owns neq x 64 doubles
-  const size_t ws_waves = std::max((size_t)grid_s * tgr_surr_block_waves(), (size_t)grid_p * tgr_step_block_waves());
+  int grid_p = 0;
+  bool lanes_p = false;
+  // the workspace serves both kernels: each wave of the guided kernel owns
+  // neq x 64 doubles, the plain step on the rejected cells its own shape
+  const size_t ws_bytes = std::max((size_t)grid_s * tgr_surr_block_waves() * c->neq * 64 * sizeof(double),
+                                   step_shape(c, p.ncell, &grid_p, &lanes_p));
   // every buffer sized before the first launch (a resize frees memory)
-  if ((rc = c->b_ws.ensure(ws_waves * c->neq * 64 * sizeof(double))) ||
+  if ((rc = c->b_ws.ensure(ws_bytes)) ||
       (rc = c->b_x.ensure((size_t)rows * c->net_in_pad * 2)) ||
       (rc = c->b_guess.ensure((size_t)nst * rows * c->net_out_pad * 4)) ||
       (rc = c->b_rej.ensure((size_t)p.ncell * sizeof(int))))
