@@ -16,8 +16,9 @@ max species error <= 1e-6 against an FP64 reference: the GPU plain step at
 rtol 1e-10 / atol 1e-16 on the whole block (that solver matches the CPU
 oracle to rel 1e-8 in the parity tests), cross-checked by the CPU oracle at
 the same tolerances on a stratified sample (every stiffness class).  The
-guided path is the headline when it meets the bound; its max / RMS species
-and max T errors, overall and per class, are reported either way.
+guided path is the headline when it meets the bound and is faster than the
+plain step (else the plain step is, over the same K steps); its max / RMS
+species and max T errors, overall and per class, are reported either way.
 
 Prints one JSON line: cell-steps/s, the FP64 roofline of the headline path's
 dominant kernel (algorithmic flops from the solver counters / kernel time,
@@ -533,24 +534,28 @@ def main():
         __import__("tiger_nga_amd.surrogate", fromlist=["model_path"]).model_path(mech))
     # the plain step: K steps when it is the headline, else a few (side measurement)
     k_plain = max(1, min(a.steps, a.side_steps)) if have_guided else a.steps
-    stats = None
-    _progress("plain step, %s, %d + %d steps" % (cells_label, a.warmup, k_plain))
-    for _ in range(max(1, a.warmup)):
-        S_d.copy_(S0_d)
-        stats = fc.source_chemistry(R_d, S_d, g, dt, P, P, HR_gp=HR_d, chemsrc_gp=CS_d, stream=sp, sync=True)
-    torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(k_plain)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(k_plain):
-        S_d.copy_(S0_d)
-        evs[k][0].record(stream)
-        fc.source_chemistry(R_d, S_d, g, dt, P, P, HR_gp=HR_d, chemsrc_gp=CS_d, stream=sp, sync=False)
-        evs[k][1].record(stream)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    kms = [e0.elapsed_time(e1) for e0, e1 in evs]
-    kernel_s = sum(kms) / len(kms) / 1e3
+
+    def time_plain(k_steps):
+        st = None
+        _progress("plain step, %s, %d + %d steps" % (cells_label, a.warmup, k_steps))
+        for _ in range(max(1, a.warmup)):
+            S_d.copy_(S0_d)
+            st = fc.source_chemistry(R_d, S_d, g, dt, P, P, HR_gp=HR_d, chemsrc_gp=CS_d, stream=sp, sync=True)
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(k_steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(k_steps):
+            S_d.copy_(S0_d)
+            evs[k][0].record(stream)
+            fc.source_chemistry(R_d, S_d, g, dt, P, P, HR_gp=HR_d, chemsrc_gp=CS_d, stream=sp, sync=False)
+            evs[k][1].record(stream)
+        torch.cuda.synchronize()
+        el_ = time.perf_counter() - t0
+        kms = [e0.elapsed_time(e1) for e0, e1 in evs]
+        return el_, sum(kms) / len(kms) / 1e3, st
+
+    el, kernel_s, stats = time_plain(k_plain)
     info = fc.info
     F = flops.step_flops(info, stats)
     achieved = F / kernel_s
@@ -662,9 +667,22 @@ def main():
             rg["traffic_source"] = trs["source"]
         sur["guided_roofline"] = rg
         # the guided path is the headline when it meets the metric's error bound
-        # against the FP64 reference (the plain step's max species error beside)
-        ok = ref_rows is not None and sur["vs_reference"]["max_species_err"] <= 1e-6
-        sur["meets_bound"] = ok
+        # against the FP64 reference and is faster than the plain step; else
+        # the plain step is, timed over the full K steps
+        bound = ref_rows is not None and sur["vs_reference"]["max_species_err"] <= 1e-6
+        sur["meets_bound"] = bound
+        sur["faster_than_plain"] = sur["value"] > plain_value
+        sur["speedup_vs_plain"] = sur["value"] / plain_value
+        ok = bound and sur["faster_than_plain"]
+        if not ok and k_plain < a.steps:
+            el, kernel_s, _ = time_plain(a.steps)
+            plain_value = cells * a.steps / el
+            out["value"] = plain_value
+            out["steps"] = a.steps
+            out["ms_per_step"] = el / a.steps * 1e3
+            out["roofline"]["kernel_ms"] = kernel_s * 1e3
+            out["roofline"]["achieved"] = F / kernel_s / 1e12
+            out["roofline"]["frac"] = F / kernel_s / PEAK_FP64
         if ok:
             out["plain_value"] = plain_value
             out["plain_steps"] = k_plain

@@ -62,6 +62,10 @@ SURR_KEEP = (1.0, 1.5)
 # slower than SURR_JAC_THETA with a Jacobian of an earlier state)
 SURR_MAX_JAC = (1, 4)
 SURR_JAC_THETA = 0.3
+# a guided-solve configuration is recorded only when it beats the plain step
+# by this factor on the tuning states (a configuration that rejects nearly
+# every cell runs the plain step behind a failed solve and times like it)
+SURR_MIN_GAIN = 1.03
 
 
 def _device_name():
@@ -231,6 +235,10 @@ def tune(mech: str, cells: int = 32 ** 3, seed: int = 7, reps: int = 2, log=prin
                 if max(errs) <= MAX_SPECIES_ERR and max(terrs) <= MAX_T_ERR and (best is None or t_tot < best[0]):
                     best = (t_tot, adaptive, nsub, grade, att, guess, h0f, mj, keep, tol)
         rec["measured"]["surrogate"] = st_all
+        best = guided_choice(best, sets[0][5] + sets[1][5])
+        if not best:
+            rec["measured"]["surrogate_choice"] = ("none: no configuration within the error bounds is %.2fx faster "
+                                                   "than the plain step" % SURR_MIN_GAIN)
         if best:
             (_, rec["surr_adaptive"], rec["surr_nsub"], rec["surr_grade"], rec["surr_max_att"], rec["surr_guess"],
              rec["surr_h0_fac"], rec["surr_max_jac"], rec["surr_keep"], rec["surr_tol_y"]) = best
@@ -239,6 +247,14 @@ def tune(mech: str, cells: int = 32 ** 3, seed: int = 7, reps: int = 2, log=prin
         fc.set_surrogate(None, None, None, None)
     fc.close()
     return rec
+
+
+def guided_choice(best, plain_s):
+    """The fastest configuration within the error bounds (time first), kept
+    only when it beats the plain step's time by SURR_MIN_GAIN."""
+    if best is None or best[0] * SURR_MIN_GAIN > plain_s:
+        return None
+    return best
 
 
 def surr_name(adaptive, nsub, grade, att, guess, h0f, mj, keep, tol):
