@@ -310,3 +310,95 @@ extern "C" int emu_lane_step(const char* mech, long ncell, double* sol, double d
   }
   return 0;
 }
+
+namespace {
+
+// Dense solves of the wave solver on n matrices a[n][NEQ][NEQ] (row-major)
+// with right-hand sides b[n][NEQ]: Bdf::getrf + getrs into xlu, Bdf::getri +
+// getri_solve into xinv; info[n][2] the two factorisations' return codes.
+template <int NEQ>
+struct LinRun {
+  long n;
+  const double *a, *b;
+  double *xlu, *xinv;
+  int* info;
+  tgr::DevMech m{};
+  tgr::RhsLds<NEQ> rl;
+  std::vector<double> ws;
+  tgr_emu::Wave wave;
+  int finished = 0;
+
+  static void lane_main(unsigned hi, unsigned lo, int l) {
+    LinRun* r = (LinRun*)(((uintptr_t)hi << 32) | (uintptr_t)lo);
+    for (long c = 0; c < r->n; ++c) {
+      tgr::Bdf<NEQ> s(r->m, r->m.tab, &r->rl, r->ws.data(), 1.0e5, 1.0e-6, 1.0e-12);
+      const double bv = l < NEQ ? r->b[c * NEQ + l] : 0.0;
+      for (int k = 0; k < NEQ; ++k) s.M[k] = l < NEQ ? r->a[(c * NEQ + l) * NEQ + k] : 0.0;
+      const int i1 = s.getrf();
+      const double x1 = i1 ? 0.0 : s.getrs(bv);
+      for (int k = 0; k < NEQ; ++k) s.M[k] = l < NEQ ? r->a[(c * NEQ + l) * NEQ + k] : 0.0;
+      const int i2 = s.getri();
+      // two right-hand sides on one inverse, as the Newton iterations use it
+      const double x2 = i2 ? 0.0 : s.getri_solve(bv);
+      const double x3 = i2 ? 0.0 : s.getri_solve(bv);
+      if (l < NEQ) {
+        r->xlu[c * NEQ + l] = x1;
+        r->xinv[c * NEQ + l] = (x2 == x3) ? x2 : NAN;
+      }
+      if (l == 0) {
+        r->info[2 * c] = i1;
+        r->info[2 * c + 1] = i2;
+      }
+    }
+    r->finished++;
+    if (r->finished == 64) {
+      setcontext(&r->wave.home);
+    } else {
+      const int nx = (l + 1) & 63;
+      tgr_emu::tl_lane = nx;
+      setcontext(&r->wave.ctx[nx]);
+    }
+  }
+};
+
+template <int NEQ>
+void run_lin(long n, const double* a, const double* b, double* xlu, double* xinv, int* info) {
+  LinRun<NEQ>* r = new LinRun<NEQ>();
+  r->n = n;
+  r->a = a;
+  r->b = b;
+  r->xlu = xlu;
+  r->xinv = xinv;
+  r->info = info;
+  r->ws.assign((size_t)NEQ * tgr::kWave, 0.0);
+  const size_t stack = 1 << 20;
+  std::vector<char> stacks(stack * 64);
+  tgr_emu::tw = &r->wave;
+  for (int l = 0; l < 64; ++l) {
+    getcontext(&r->wave.ctx[l]);
+    r->wave.ctx[l].uc_stack.ss_sp = stacks.data() + stack * l;
+    r->wave.ctx[l].uc_stack.ss_size = stack;
+    r->wave.ctx[l].uc_link = nullptr;
+    const uintptr_t u = (uintptr_t)r;
+    makecontext(&r->wave.ctx[l], (void (*)())LinRun<NEQ>::lane_main, 3, (unsigned)(u >> 32),
+                (unsigned)(u & 0xffffffffu), l);
+  }
+  tgr_emu::tl_lane = 0;
+  swapcontext(&r->wave.home, &r->wave.ctx[0]);
+  delete r;
+}
+
+}  // namespace
+
+// The wave solver's two dense solves (LU and Gauss-Jordan inverse) for neq
+// 10, 37, 40 or 46 (the shipped mechanisms' sizes).
+extern "C" int emu_linsolve(int neq, long n, const double* a, const double* b, double* xlu, double* xinv,
+                            int* info) {
+  switch (neq) {
+    case 10: run_lin<10>(n, a, b, xlu, xinv, info); return 0;
+    case 37: run_lin<37>(n, a, b, xlu, xinv, info); return 0;
+    case 40: run_lin<40>(n, a, b, xlu, xinv, info); return 0;
+    case 46: run_lin<46>(n, a, b, xlu, xinv, info); return 0;
+    default: return -1;
+  }
+}

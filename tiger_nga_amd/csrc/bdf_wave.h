@@ -174,6 +174,7 @@ struct Bdf {
     n = NEQ;
     act = lane < NEQ;
     qmax = kQmaxAbs;
+    pivl = lane;
   }
 
   __device__ __forceinline__ double f(double yv) { return rhs_call<NEQ>(tab, S, m.tmid, act ? yv : 0.0, p_gp); }
@@ -244,6 +245,67 @@ struct Bdf {
       xo = (lane == k) ? xk : xo;
     }
     return xo;
+  }
+
+  // Gauss-Jordan inverse of M in place, partial pivoting, for the one-step
+  // solves (about twenty solves per factorisation there: each one is then a
+  // matrix-vector product of independent broadcasts instead of two dependent
+  // triangular sweeps).  Rows stay in their lanes.  Step k exchanges unknown
+  // k with equation p_k, the not yet pivoted row of largest |a_ik|: with
+  // r = 1 / a_pk, row p_k becomes r in column k and -r a_pj elsewhere, every
+  // other row i becomes a_ik r in column k and a_ij - a_ik r a_pj elsewhere
+  // (the pivot row through LDS).  The columns rotate by one per step (the
+  // updated column k+1 lands in M[0], column k's new values in M[NEQ-1]), so
+  // that the pivot column is always M[0] and the steps run as a loop with
+  // constant register indices; after the NEQ steps every column is back in
+  // place.  Row p_k then expresses x_k with the coefficient of b_{p_j} in
+  // column j; lane k receives row p_k, and pivl (lane k) holds p_k.  Returns
+  // 0, or k + 1 when column k has no nonzero candidate.
+  __device__ __forceinline__ int getri() {
+    TGR_LDS double* prow = S->q;
+    bool cand = act;
+#pragma unroll 1
+    for (int k = 0; k < NEQ; ++k) {
+      const double a = cand ? fabs(M[0]) : -1.0;
+      const double amax = (wave_max(a));
+      if (!(amax > 0.0)) return k + 1;
+      const uint64_t bal = __ballot(cand && a == amax);
+      const int p = __builtin_amdgcn_readfirstlane(bal ? (int)__builtin_ctzll(bal) : 0);
+      const bool isp = lane == p;
+      if (isp) {
+        cand = false;
+#pragma unroll
+        for (int c = 0; c < NEQ; ++c) prow[c] = M[c];
+      }
+      wave_sync();
+      const double r = 1.0 / prow[0];
+      if (lane == k) pivl = p;
+      // pivot row: 0 * a_pc - r a_pc; other rows: a_ic - (a_ik r) a_pc (the
+      // products by 0 and 1 are exact)
+      const double fk = (isp ? 1.0 : M[0]) * r;
+      const double g = isp ? 0.0 : 1.0;
+#pragma unroll
+      for (int c = 0; c + 1 < NEQ; ++c) M[c] = fma(-fk, prow[c + 1], g * M[c + 1]);
+      M[NEQ - 1] = fk;
+      wave_sync();
+    }
+#pragma unroll
+    for (int c = 0; c < NEQ; ++c) M[c] = __shfl(M[c], pivl);
+    return 0;
+  }
+
+  // x = A^-1 b with the inverse from getri (b in equation order, x in
+  // unknown order; 0 on the lanes past NEQ); two partial sums
+  __device__ __forceinline__ double getri_solve(double b) {
+    const double bp = __shfl(b, pivl);  // lane j: b_{p_j}
+    double x0 = 0.0, x1 = 0.0;
+#pragma unroll
+    for (int j = 0; j + 1 < NEQ; j += 2) {
+      x0 = fma(M[j], bcast(bp, j), x0);
+      x1 = fma(M[j + 1], bcast(bp, j + 1), x1);
+    }
+    if constexpr (NEQ % 2) x0 = fma(M[NEQ - 1], bcast(bp, NEQ - 1), x0);
+    return act ? x0 + x1 : 0.0;
   }
 
   // difference-quotient Jacobian, column j perturbs component j
@@ -814,7 +876,7 @@ struct Bdf {
   // stiffly accurate SDIRK method of order 3 (Alexander 1977; gamma the root
   // of x^3 - 3x^2 + 3x/2 - 1/6 in (1/6, 1/2), c = (gamma, (1+gamma)/2, 1),
   // a21 = (1-gamma)/2, b = (-(6g^2-16g+1)/4, (6g^2-20g+5)/4, gamma)) with one
-  // difference-quotient Jacobian at the initial state and one LU of
+  // difference-quotient Jacobian at the initial state and one inverse of
   // I - gamma h J per sub-step.  Stage values are solved by simplified Newton
   // starting, with use_guess, from the previous stage value moved by the
   // network's change guess(i, j) - guess(previous stage) (sub-step i, stage j,
@@ -834,7 +896,7 @@ struct Bdf {
   static constexpr double kSdBh1 = 1.0 - kSdBh2;
 
   // Simplified Newton for one stage value z = base + gh f(z), the iteration
-  // matrix's LU factors in M.  The iterate's remaining error is about
+  // matrix's inverse in M (getri).  The iterate's remaining error is about
   // theta / (1 - theta) times the last correction, theta the contraction rate
   // |d_k| / |d_k-1|, so the iteration stops when max(1, theta / (1 - theta))
   // |d| <= ntol (in acceptance tolerances): never looser than |d| <= ntol, and
@@ -850,7 +912,7 @@ struct Bdf {
       const double fz = f(z);
       c_nfe++;
       const double r = act ? z - base - gh * fz : 0.0;
-      const double d = getrs(-r);
+      const double d = getri_solve(-r);
       nit++;
       if (act) z += d;
       const double dn = wave_max(act ? fabs(d) * wacc : 0.0);
@@ -871,9 +933,9 @@ struct Bdf {
   // h0_fac / max_k |f_k(y0)| wacc_k within [dt/1024, dt]), each step's error
   // estimate e (as above) accepts the step when e <= 1 and scales the next
   // step by min(grow, max(0.2, 0.9 e^(-1/3))) -- after an accepted step a
-  // factor in [1, keep) keeps h, so that the LU factors of I - gamma h J are
+  // factor in [1, keep) keeps h, so that the inverse of I - gamma h J is
   // reused -- a failed step is retried with the smaller h; the last step is
-  // cut to end at dt.  One Jacobian at the initial state, one LU per change
+  // cut to end at dt.  One Jacobian at the initial state, one inverse per change
   // of h.  gat(t) is the network's prediction of
   // the state at time t; each stage's Newton starts from the previous stage
   // value moved by the prediction's change between the two stage times;
@@ -962,7 +1024,7 @@ struct Bdf {
         }
         c_nlu++;
         h_lu = -1.0;
-        if (getrf()) {
+        if (getri()) {
           hh *= 0.25;  // singular iteration matrix: retry with a smaller step
           why = 1;
           continue;
@@ -1018,7 +1080,7 @@ struct Bdf {
       }
       const double k3 = act ? (z - base3) * rg : 0.0;
       const double e =
-          getrs(act ? hh * ((kSdB1 - kSdBh1) * k1 + (kSdB2 - kSdBh2) * k2 + kSdG * k3) : 0.0);
+          getri_solve(act ? hh * ((kSdB1 - kSdBh1) * k1 + (kSdB2 - kSdBh2) * k2 + kSdG * k3) : 0.0);
       const double en = wave_max(act ? fabs(e) * wacc : 0.0);
       if (!isfinite(en)) return 1;
       double fac = fmin(grow, fmax(0.2, 0.9 * rpow(fmax(en, 1.0e-10), -1.0 / 3.0)));
@@ -1086,7 +1148,7 @@ struct Bdf {
           M[c] = v;
         }
         c_nlu++;
-        if (getrf()) return 1;
+        if (getri()) return 1;
         h_lu = hh;
       }
       const double rg = 1.0 / gh;
@@ -1102,7 +1164,7 @@ struct Bdf {
       if (!sdirk_newton(z, base3, gh, wacc, nmax, ntol, nit, theta, thmax)) return 1;
       const double k3 = act ? (z - base3) * rg : 0.0;
       const double e =
-          getrs(act ? hh * ((kSdB1 - kSdBh1) * k1 + (kSdB2 - kSdBh2) * k2 + kSdG * k3) : 0.0);
+          getri_solve(act ? hh * ((kSdB1 - kSdBh1) * k1 + (kSdB2 - kSdBh2) * k2 + kSdG * k3) : 0.0);
       const double en = wave_max(act ? fabs(e) * wacc : 0.0);
       nst++;
       if (!(en <= 1.0)) return 2;
