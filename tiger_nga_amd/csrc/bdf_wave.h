@@ -235,16 +235,16 @@ struct Bdf {
 #pragma unroll
     for (int k = 0; k < NEQ - 1; ++k) {
       const double yk = bcast(b, k);
-      if (lane > k && lane < NEQ) b = fma(-M[k], yk, b);
+      if (lane > k) b = fma(-M[k], yk, b);  // lanes past NEQ: never read
     }
-    double xo = 0.0;
 #pragma unroll
     for (int k = NEQ - 1; k >= 0; --k) {
       const double xk = bcast(b, k) * bcast(rdk, k);
       if (lane < k) b = fma(-M[k], xk, b);
-      xo = (lane == k) ? xk : xo;
     }
-    return xo;
+    // lane k's b is final from step k on (later steps update lanes below
+    // it): its x_k is the same product the step broadcast
+    return act ? b * rdk : 0.0;
   }
 
   // Gauss-Jordan inverse of M in place, partial pivoting, for the one-step
@@ -295,16 +295,22 @@ struct Bdf {
   }
 
   // x = A^-1 b with the inverse from getri (b in equation order, x in
-  // unknown order; 0 on the lanes past NEQ); two partial sums
+  // unknown order; 0 on the lanes past NEQ): b, in pivot order, goes through
+  // the idle LDS scratch and comes back to every lane as broadcast reads (LDS
+  // instructions instead of a pair of v_readlane per component); two partial
+  // sums
   __device__ __forceinline__ double getri_solve(double b) {
-    const double bp = __shfl(b, pivl);  // lane j: b_{p_j}
+    TGR_LDS double* v = S->q;
+    v[lane] = __shfl(b, pivl);  // lane j: b_{p_j}
+    wave_sync();
     double x0 = 0.0, x1 = 0.0;
 #pragma unroll
     for (int j = 0; j + 1 < NEQ; j += 2) {
-      x0 = fma(M[j], bcast(bp, j), x0);
-      x1 = fma(M[j + 1], bcast(bp, j + 1), x1);
+      x0 = fma(M[j], v[j], x0);
+      x1 = fma(M[j + 1], v[j + 1], x1);
     }
-    if constexpr (NEQ % 2) x0 = fma(M[NEQ - 1], bcast(bp, NEQ - 1), x0);
+    if constexpr (NEQ % 2) x0 = fma(M[NEQ - 1], v[NEQ - 1], x0);
+    wave_sync();
     return act ? x0 + x1 : 0.0;
   }
 
