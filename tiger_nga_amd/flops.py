@@ -90,10 +90,12 @@ def properties_flops(nsp: int) -> float:
 def guided_flops(info: dict, sstats: dict, nsub: int = 0, split: bool = False) -> float:
     """Algorithmic flops of the surrogate step's guided solve (before the plain
     fallback) from its counters: every right-hand side (Newton and
-    difference-quotient columns), each LU of I - gamma h J, one filtered
-    error estimate per sub-step taken, a triangular solve and update per
-    Newton iteration, and the enthalpy check / element projection (taken as
-    60 neq per cell).  Without the factorisation counter, one LU and one
+    difference-quotient columns), each factorisation of I - gamma h J counted
+    as an LU (2 n^2 + 2/3 n^3: the kernel forms the Gauss-Jordan inverse, n^3
+    multiply-adds, and the excess over the LU is not counted), one filtered
+    error estimate per sub-step taken, a solve (2 n^2, as two triangular
+    sweeps or one product with the inverse) and update per Newton iteration,
+    and the enthalpy check / element projection (taken as 60 neq per cell).  Without the factorisation counter, one LU and one
     estimate per counted sub-step (or nsub per attempted cell).  split: each
     Jacobian's nsp species columns are fixed-temperature evaluations
     (rhs_flops_cached)."""
