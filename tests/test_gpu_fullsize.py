@@ -1,0 +1,124 @@
+"""The full GRI-3.0 block of the benchmark, 128^3 = 2,097,152 cells of the
+uneven-stiffness distribution (70 % near equilibrium, 20 % ignition front,
+10 % cold; another seed than the benchmark's), through the device entry point,
+held to properties that do not depend on the block size:
+
+* every cell's element inventory sum_k E_ek Y_k / W_k is an exact invariant
+  of finitechem_compute_rhs (each reaction conserves atoms; rho Y_k / W_k are
+  the concentrations the production rates move), and a Newton/BDF solve keeps
+  linear invariants to its convergence level;
+* every cell's mixture enthalpy sum_k Y_k h_k(T) is conserved by the
+  constant-pressure step (the temperature row is its derivative), to the
+  integration tolerance;
+* the surrogate-guided step (tuned options, forced on every call) is within
+  the metric's 1e-6 of the plain step in every mass fraction, and its
+  accepted cells conserve the elements to rounding (they are projected).
+
+The enthalpies come from the oracle's NASA-7 evaluation on a sample of cells.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from tiger_nga_amd import inputs
+from tiger_nga_amd.finitechem import Grid
+from tiger_nga_amd.mech import meta
+
+pytestmark = pytest.mark.gpu
+
+NX = 128
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    assert torch.cuda.is_available(), "GPU test without a HIP device"
+    return torch
+
+
+def _empty(torch, shape):
+    strides, acc = [], 1
+    for s in shape:
+        strides.append(acc)
+        acc *= s
+    return torch.empty_strided(shape, tuple(strides), dtype=torch.float64, device="cuda")
+
+
+def _rows(torch, S, nvar):
+    """[cells, nvar] of a Fortran-strided (nx, ny, nz, nscalar) device field, i fastest."""
+    return torch.stack([S[..., k].permute(2, 1, 0).reshape(-1) for k in range(nvar)], dim=1)
+
+
+def _enthalpy(oracle, mech, rows, nsp):
+    h = np.empty(rows.shape[0])
+    for c in range(rows.shape[0]):
+        hk, _ = oracle.thermo(mech, float(rows[c, nsp]), nsp)
+        h[c] = float(rows[c, :nsp] @ hk)
+    return h
+
+
+def test_full_block_invariants_plain_and_guided(torch_dev, oracle):
+    torch = torch_dev
+    from tiger_nga_amd import autotune, surrogate
+    from tiger_nga_amd.finitechem import FiniteChem
+    mech, Y, T, rho, P, dt = inputs.states("gri30_uneven", seed=3)
+    n = Y.shape[0]
+    assert n == NX ** 3
+    nsp = Y.shape[1]
+    RHO_, SC_, isc_sc, isc_T = inputs.fortran_fields(Y, T, rho, NX, NX, NX)
+    g = Grid.block(NX, NX, NX, SC_.shape[3], isc_sc, isc_T)
+    R = _empty(torch, RHO_.shape)
+    R.copy_(torch.from_numpy(RHO_))
+    S0 = _empty(torch, SC_.shape)
+    S0.copy_(torch.from_numpy(SC_))
+    del RHO_, SC_
+    S = _empty(torch, tuple(S0.shape))
+    fc = FiniteChem(mech, 1e-6, 1e-12, 10000)
+    autotune.apply(fc, n)
+    S.copy_(S0)
+    st = fc.source_chemistry(R, S, g, dt, P, P)
+    assert st["failed"] == 0 and st["cells"] == n
+    plain = _rows(torch, S, nsp + 1)
+    init = _rows(torch, S0, nsp + 1)
+    path = surrogate.model_path(mech)
+    assert os.path.exists(path), path
+    fc.load_surrogate(path)
+    fc.set_option("surr_probe", 0)
+    S.copy_(S0)
+    st_g, ss = fc.source_chemistry_surrogate(R, S, g, dt, P, P)
+    fc.close()
+    assert st_g["failed"] == 0 and ss["accepted"] + ss["rejected"] == n
+    guided = _rows(torch, S, nsp + 1)
+    del S, S0, R
+
+    # element inventories (mol of each element per kg of mixture)
+    EW = torch.from_numpy(np.ascontiguousarray(meta.element_matrix(mech) / meta.molar_mass(mech)[None, :])).cuda()
+    EW = EW[EW.abs().sum(dim=1) > 0].t().contiguous()  # [nsp, elements present]
+    e0 = init[:, :nsp] @ EW
+    scale = e0.abs().max(dim=0).values
+    for name, rows in (("plain", plain), ("guided", guided)):
+        assert bool(torch.isfinite(rows).all()), name
+        assert float(rows[:, nsp].min()) > 200.0 and float(rows[:, nsp].max()) < 4000.0, name
+        de = ((rows[:, :nsp] @ EW - e0).abs() / scale).max().item()
+        print("\n%s: max element-inventory change %.2e of the element's largest inventory" % (name, de))
+        assert de <= 1e-9, (name, de)
+
+    # guided against plain, every cell
+    dY = (guided[:, :nsp] - plain[:, :nsp]).abs().max().item()
+    dT = (guided[:, nsp] - plain[:, nsp]).abs().max().item()
+    print("guided vs plain: accepted %.4f, max |dY| %.2e, max |dT| %.2e K" % (ss["accepted"] / n, dY, dT))
+    assert ss["accepted"] / n >= 0.95
+    assert dY <= 1e-6 and dT <= 0.05
+
+    # mixture enthalpy on a sample spanning the stiffness classes
+    rng = np.random.default_rng(1)
+    pick = np.sort(rng.choice(n, 3000, replace=False))
+    ix = torch.from_numpy(pick).cuda()
+    i0, ip, ig = (r[ix].cpu().numpy() for r in (init, plain, guided))
+    h0 = _enthalpy(oracle, mech, i0, nsp)
+    cp_scale = 1.5e3  # J/kg/K, order of the mixtures' c_p
+    for name, rows, tol_K in (("plain", ip, 0.01), ("guided", ig, 0.02)):
+        dh = np.abs(_enthalpy(oracle, mech, rows, nsp) - h0) / cp_scale
+        print("%s: max enthalpy change over the sample %.2e K-equivalent" % (name, dh.max()))
+        assert dh.max() <= tol_K, (name, dh.max())

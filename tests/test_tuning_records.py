@@ -35,7 +35,7 @@ def test_recorded_guided_choice_follows_the_rules(key):
         return
     name = autotune.surr_name(rec["surr_adaptive"], rec["surr_nsub"], rec["surr_grade"], rec["surr_max_att"],
                               rec["surr_guess"], rec["surr_h0_fac"], rec["surr_max_jac"], rec["surr_keep"],
-                              rec["surr_tol_y"])
+                              rec["surr_tol_y"], rec.get("surr_newton_tol"))
     r = m["surrogate"][name]
     assert max(r["max_species_err_generated"], r["max_species_err_relaxed"]) <= autotune.MAX_SPECIES_ERR
     assert r["max_T_err_K"] <= autotune.MAX_T_ERR
