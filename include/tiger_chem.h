@@ -275,12 +275,15 @@ int tgr_chem_load_surrogate(tgr_chem *ctx, const char *path);
 
 /* Real-valued knobs of the surrogate step: "surr_tol_y" (default 2.5e-7: the
  * adaptive schedule's error grows with its number of steps, each within the
- * threshold; at this setting its largest species error against a 1e-10
- * tolerance solve stays below 9e-7 on the H2, GRI-3.0, EthyRedF and DME39
- * benchmark states, at 1e-6 it reaches 3.6e-6 on H2 -- the tuning record
- * holds the largest safe value per mechanism, 1e-6 for GRI-3.0),
+ * threshold, mostly through the stage solves' Newton convergence error; at
+ * the defaults its largest species error against a 1e-10 tolerance solve on
+ * 2 x 32^3 benchmark states is 1.0e-7 (GRI-3.0), 6.6e-8 (EthyRedF), 2.6e-7
+ * (DME39) and 8.3e-7 (H2) -- the tuning record holds the fastest safe values
+ * per mechanism, for GRI-3.0 5e-7 with a Newton stop of 0.03),
  * "surr_tol_t" (K, default 1e-2), "surr_newton_tol" (Newton stops when the
- * correction is below this fraction of the acceptance tolerance, default 0.1),
+ * correction is below this fraction of the acceptance tolerance, default 0.03;
+ * at 0.1 the convergence error of the stage solves adds up over the steps to
+ * two or three times the error at 0.03),
  * "surr_grade" (ratio of consecutive sub-step sizes, default 2),
  * "surr_min_accept" (adaptive-use threshold, default 0.15), "surr_h0_fac"
  * (adaptive schedule: first step h0_fac / max_k |f_k(y0)| / tol_k within

@@ -116,7 +116,7 @@ struct tgr_chem {
   double surr_h0_fac = 0.0, surr_grow = 10.0, surr_keep = 1.5, surr_jac_theta = 0.3;
   int surr_max_jac = 4;
   double surr_min_accept = 0.15, surr_last_accept = 1.0;
-  double surr_newton_tol = 0.1, surr_tol_y = 2.5e-7, surr_tol_t = 1.0e-2, surr_grade = 2.0;
+  double surr_newton_tol = 0.03, surr_tol_y = 2.5e-7, surr_tol_t = 1.0e-2, surr_grade = 2.0;
   hipEvent_t ev2 = nullptr;
 };
 
