@@ -201,7 +201,7 @@ int tgr_chem_temperature_device(tgr_chem *ctx, int64_t n, const double *Y, const
  * controls the step size by the embedded error estimate (filtered by
  * (I - gamma h J)^-1, accepted when |e_k| <= surr_tol_y in Y and surr_tol_t in
  * T; next step x min(surr_grow, max(0.2, 0.9 e^-1/3)), where a factor in
- * [1, surr_keep) keeps h and reuses the LU factors of I - gamma h J; a stage
+ * [1, surr_keep) keeps h and reuses the inverse of I - gamma h J; a stage
  * whose Newton iteration does not converge retries the step at h/4) from a
  * first step of
  * dt (r - 1) / (r^n - 1), r = "surr_grade", n = "surr_nsub", for at most
@@ -233,7 +233,7 @@ typedef struct {
   int64_t newton_iters, rhs_evals; /* of the guided solve (Jacobian columns included) */
   int64_t rej_newton, rej_error, rej_enthalpy; /* why cells were rejected */
   int64_t substeps;               /* sub-steps taken by the guided solve (accepted ones, until a cell stops) */
-  int64_t factorizations;         /* LU factorisations of I - gamma h J in the guided solve */
+  int64_t factorizations;         /* factorisations (Gauss-Jordan inverses) of I - gamma h J in the guided solve */
   int64_t jacobians;              /* Jacobian evaluations of the guided solve (1 per cell, more with refreshes) */
   double surrogate_ms, fallback_ms; /* device time of features + network + guided solve, of the plain step */
   int guided;                     /* 1: the guided solve ran; 0: the adaptive policy sent the call to the plain step */
@@ -289,8 +289,8 @@ int tgr_chem_load_surrogate(tgr_chem *ctx, const char *path);
  * (adaptive schedule: first step h0_fac / max_k |f_k(y0)| / tol_k within
  * [dt/1024, dt] instead of the first sub-step; default 0 = off), "surr_grow"
  * (adaptive: largest step growth, 1.5..100, default 10), "surr_keep" (adaptive:
- * growth factors below this keep the step and its LU factors, 1..4, default
- * 1.5; on GRI-3.0 64^3 this halves the LU count and makes the guided step 9 %
+ * growth factors below this keep the step and its factors, 1..4, default
+ * 1.5; on GRI-3.0 64^3 this halves the factorisation count and makes the guided step 9 %
  * faster at unchanged error), "surr_jac_theta" (adaptive: a step attempt whose
  * Newton iterations contract slower than this rate, or fail, with a Jacobian
  * of an earlier state re-evaluates the Jacobian at the current state before

@@ -68,7 +68,7 @@ SURR_SCHEDULES = ((1, 8, 2.0, 48), (1, 4, 1.0, 32), (1, 8, 3.0, 64), (1, 8, 3.0,
 SURR_TOLS = (1.5e-7, 2e-7, 2.5e-7, 5e-7, 1e-6, 2e-6)
 # first step of the adaptive schedule from the initial rate (0: the schedule's)
 SURR_H0 = (0.0, 16.0)
-# adaptive schedule: growth factors in [1, keep) keep the step and its LU
+# adaptive schedule: growth factors in [1, keep) keep the step and its
 # factors (1.0: every accepted step takes its new size)
 SURR_KEEP = (1.5,)
 # Newton stop of the stage solves in acceptance tolerances (surr_newton_tol):
