@@ -14,7 +14,8 @@
 // mixture enthalpy exactly) agrees with the solved temperature; the written
 // temperature is the enthalpy-consistent one.  Every other cell is appended to
 // a device list that the plain integrator (step_kernel) then processes, so
-// the plain FP64 solver is both the fallback and the judge.
+// the plain FP64 solver is both the fallback and the reference the
+// acceptance thresholds are measured against.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
