@@ -1,7 +1,8 @@
-"""The full GRI-3.0 block of the benchmark, 128^3 = 2,097,152 cells of the
-uneven-stiffness distribution (70 % near equilibrium, 20 % ignition front,
-10 % cold; another seed than the benchmark's), through the device entry point,
-held to properties that do not depend on the block size:
+"""The full 128^3 = 2,097,152-cell blocks of the GRI-3.0 benchmark (the
+uneven-stiffness distribution: 70 % near equilibrium, 20 % ignition front,
+10 % cold) and of DME39 at 10 atm, of another seed than the benchmark's,
+through the device entry point, held to properties that do not depend on the
+block size:
 
 * every cell's element inventory sum_k E_ek Y_k / W_k is an exact invariant
   of finitechem_compute_rhs (each reaction conserves atoms; rho Y_k / W_k are
@@ -58,11 +59,17 @@ def _enthalpy(oracle, mech, rows, nsp):
     return h
 
 
-def test_full_block_invariants_plain_and_guided(torch_dev, oracle):
+# smallest accepted fraction of the guided solve on each block (DME39's
+# tuned attempt budget hands the cells that need more steps to the plain step)
+MIN_ACCEPT = {"gri30_uneven": 0.95, "dme39": 0.85}
+
+
+@pytest.mark.parametrize("workload", ["gri30_uneven", "dme39"])
+def test_full_block_invariants_plain_and_guided(torch_dev, oracle, workload):
     torch = torch_dev
     from tiger_nga_amd import autotune, surrogate
     from tiger_nga_amd.finitechem import FiniteChem
-    mech, Y, T, rho, P, dt = inputs.states("gri30_uneven", seed=3)
+    mech, Y, T, rho, P, dt = inputs.states(workload, seed=3)
     n = Y.shape[0]
     assert n == NX ** 3
     nsp = Y.shape[1]
@@ -108,7 +115,7 @@ def test_full_block_invariants_plain_and_guided(torch_dev, oracle):
     dY = (guided[:, :nsp] - plain[:, :nsp]).abs().max().item()
     dT = (guided[:, nsp] - plain[:, nsp]).abs().max().item()
     print("guided vs plain: accepted %.4f, max |dY| %.2e, max |dT| %.2e K" % (ss["accepted"] / n, dY, dT))
-    assert ss["accepted"] / n >= 0.95
+    assert ss["accepted"] / n >= MIN_ACCEPT[workload]
     assert dY <= 1e-6 and dT <= 0.05
 
     # mixture enthalpy on a sample spanning the stiffness classes
