@@ -316,6 +316,9 @@ int tgr_chem_set_option_real(tgr_chem *ctx, const char *key, double value);
  *                 neq 10 only, whose tables leave LDS room for the second; with
  *                 "jacobian" 1 the two-wave kernel runs); the results are the
  *                 same, the autotuner times both
+ *   "props_mapping"  tgr_chem_properties_device: 1 (default) a thread per
+ *                 cell (the Wilke double sum in every lane), 0 a wavefront per
+ *                 cell (species in lanes); the same results to rounding
  *   "jacobian"    0 (default): the Jacobian by difference quotients, one
  *                 right-hand side per column, as CVODE's FCVDENSE builds it for
  *                 the reference (finitechem.f90:1028); 1: the analytic Jacobian

@@ -12,7 +12,7 @@ import pytest
 from tiger_nga_amd import inputs
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
-NSP = {"h2": 9, "gri30": 36, "dme39": 39}
+NSP = {"h2": 9, "gri30": 36, "dme39": 39, "ethyred": 45}
 
 
 @pytest.mark.parametrize("mech", ["h2", "gri30", "dme39"])
@@ -70,12 +70,17 @@ def test_lewis_scaling_and_temperature_diffusivity(oracle):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("wl", ["h2", "gri30", "dme39"])
-def test_properties_kernel_vs_oracle(wl, oracle):
+@pytest.mark.parametrize("mapping", [0, 1], ids=["wave_per_cell", "thread_per_cell"])
+@pytest.mark.parametrize("wl", ["h2", "gri30", "dme39", "ethyred"])
+def test_properties_kernel_vs_oracle(wl, mapping, oracle):
     import torch
     from tiger_nga_amd.finitechem import FiniteChem, Grid
     mech, Y, T, rho, P, dt = inputs.states(wl, n=6 * 5 * 4, seed=13)
     RHO_, SC_, isc_sc, isc_T = inputs.fortran_fields(Y, T, rho, 6, 5, 4, nover=1, extra_scalars=1)
+    # a mass fraction exactly zero and one slightly negative (the viscosity
+    # sum skips species with Y <= 0; the mixture mass clips them to 0)
+    SC_[1, 1, 1, isc_sc - 1 + 2] = 0.0
+    SC_[2, 1, 1, isc_sc - 1 + 3] = -1e-12
     nscalar = SC_.shape[3]
     g = Grid.block(6, 5, 4, nscalar, isc_sc, isc_T, nover=1)
     mask = np.zeros(g.mask_shape, dtype=np.int32, order="F")
@@ -94,6 +99,7 @@ def test_properties_kernel_vs_oracle(wl, oracle):
     R_d, V_d, D_d = fe(np.full(RHO_.shape, -7.0, order="F")), fe(np.full(RHO_.shape, -7.0, order="F")), \
         fe(np.full(SC_.shape, -7.0, order="F"))
     fc = FiniteChem(mech)
+    fc.set_option("props_mapping", mapping)
     fc.properties(S_d, g, P, mask=M_d, Lewis=torch.tensor(Le, device="cuda"), isc_zmix=isc_z, RHO_=R_d, VISC_=V_d,
                   DIFF_=D_d)
     torch.cuda.synchronize()
@@ -111,5 +117,6 @@ def test_properties_kernel_vs_oracle(wl, oracle):
                         abs(D[i, j, k, isc_z - 1] / dT - 1),
                         np.abs(D[i, j, k, isc_sc - 1:isc_sc - 1 + nsp] / dsp - 1).max())
                 worst = max(worst, e)
-    print("%s properties: max rel err %.3e over %d cells (ghosts included)" % (wl, worst, SC_[..., 0].size))
+    print("%s properties (mapping %d): max rel err %.3e over %d cells (ghosts included)"
+          % (wl, mapping, worst, SC_[..., 0].size))
     assert worst < 1e-13

@@ -111,6 +111,7 @@ struct tgr_chem {
   // every surr_probe calls (0: always guided)
   int surr_probe = 8, surr_skipped = 0;
   int surr_adaptive = 1, surr_max_att = 48, surr_split_jac = 1;
+  int props_lane = 1;  // properties kernel: 1 thread per cell, 0 wave per cell
   int surr_mode = 0;        // 0: guided solve; 1: standalone surrogate with an FP64 residual check
   double surr_sa_tol = 1.0;  // standalone: largest Newton correction accepted, in acceptance tolerances
   double surr_h0_fac = 0.0, surr_grow = 10.0, surr_keep = 1.5, surr_jac_theta = 0.3;
@@ -134,9 +135,12 @@ int build_tables(tgr_chem* c) {
   c->dm.tmid = c->t->tmid;
   // transport table: mucoeff, kOverEps, W (64 lanes each), then the Wilke
   // pair factors A(k,j) = (1/sqrt 8) (1 + W_k/W_j)^-1/2 and
-  // B(k,j) = (W_j/W_k)^1/4 at [j][k] (finitechem.f90:376-380)
+  // B(k,j) = (W_j/W_k)^1/4 at [j][k] (finitechem.f90:376-380); then, for the
+  // thread-per-cell kernel, (sqrt A(k,j), sqrt A(k,j) B(k,j)) at [k][j], zero
+  // for j = k (the reference's sum skips it)
   const int n = c->t->nsp;
-  std::vector<double> pt(192 + 2 * (size_t)n * 64, 0.0);
+  const size_t pair = 192 + 2 * (size_t)n * 64;
+  std::vector<double> pt(pair + 2 * (size_t)n * n, 0.0);
   for (int k = 0; k < n; ++k) {
     pt[k] = c->t->mucoeff[k];
     pt[64 + k] = c->t->koveps[k];
@@ -147,6 +151,11 @@ int build_tables(tgr_chem* c) {
     for (int k = 0; k < n; ++k) {
       pt[192 + (size_t)j * 64 + k] = (1.0 / std::sqrt(8.0)) * std::pow(1.0 + (c->t->W[k] / c->t->W[j]), -0.5);
       pt[192 + (size_t)n * 64 + (size_t)j * 64 + k] = std::pow(c->t->W[j] / c->t->W[k], 0.25);
+      if (j != k) {
+        const double sa = std::sqrt(pt[192 + (size_t)j * 64 + k]);
+        pt[pair + 2 * ((size_t)k * n + j)] = sa;
+        pt[pair + 2 * ((size_t)k * n + j) + 1] = sa * pt[192 + (size_t)n * 64 + (size_t)j * 64 + k];
+      }
     }
   HIP_TRY(hipMalloc(&c->d_props, pt.size() * sizeof(double)));
   HIP_TRY(hipMemcpy(c->d_props, pt.data(), pt.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -436,6 +445,9 @@ int tgr_chem_set_option(tgr_chem* c, const char* key, int64_t v) {
   } else if (!strcmp(key, "step_mapping")) {
     if (v != 0 && v != 1) return fail(TGR_E_ARG, "step_mapping must be 0 (wave per cell) or 1 (thread per cell)");
     c->step_mapping = (int)v;
+  } else if (!strcmp(key, "props_mapping")) {
+    if (v != 0 && v != 1) return fail(TGR_E_ARG, "props_mapping must be 0 (wave per cell) or 1 (thread per cell)");
+    c->props_lane = (int)v;
   } else if (!strcmp(key, "jacobian")) {
     if (v != 0 && v != 1) return fail(TGR_E_ARG, "jacobian must be 0 (difference quotients) or 1 (analytic)");
     c->jac = (int)v;
@@ -934,7 +946,7 @@ int tgr_chem_properties_device(tgr_chem* c, double p_thermo, const tgr_grid* g, 
   p.rho = RHO_;
   p.visc = VISC_;
   p.diff = DIFF_;
-  if (tgr_launch_props(&p, c->neq, (hipStream_t)stream)) return fail(TGR_E_HIP, "launch");
+  if (tgr_launch_props(&p, c->neq, c->props_lane, (hipStream_t)stream)) return fail(TGR_E_HIP, "launch");
   return 0;
 }
 

@@ -18,7 +18,7 @@ int tgr_launch_prodrates(const tgr::DevMech* m, int64_t n, const double* C, cons
 int tgr_launch_rhs(const tgr::DevMech* m, int64_t n, double p_gp, const double* y, double* f, hipStream_t s);
 int tgr_launch_temperature(const tgr::DevMech* m, int64_t n, const double* Y, const double* hmix, const double* T0,
                            double* T, int* iters, hipStream_t s);
-int tgr_launch_props(const tgr::PropsParams* p, int neq, hipStream_t s);
+int tgr_launch_props(const tgr::PropsParams* p, int neq, int lane_map, hipStream_t s);
 int tgr_launch_rhs_lanes(int neq, int64_t n, double p_gp, const double* y, double* f, hipStream_t s);
 int tgr_surr_occupancy(int neq, int adaptive);
 // waves per workgroup of the guided-solve kernel (its workspace share per block)

@@ -232,7 +232,8 @@ class FiniteChem:
             pass
 
     def set_option(self, key: str, value):
-        """Integer knobs ("grid", "rhs_mapping", "step_mapping", "step_waves", "jacobian", "max_order", "max_steps",
+        """Integer knobs ("grid", "rhs_mapping", "step_mapping", "step_waves", "props_mapping", "jacobian",
+        "max_order", "max_steps",
         "surr_nsub", "surr_guess", "surr_newton_max", "surr_probe", "surr_adaptive", "surr_max_att", "surr_max_jac",
         "surr_mode", "surr_split_jac") or real ones (REAL_OPTIONS)."""
         if key in REAL_OPTIONS:
