@@ -288,8 +288,9 @@ __global__ __launch_bounds__(256) void diff_main_kernel(DiffParams p) {
 // ---- second order (st1 = 0, st2 = 1) ----------------------------------------
 // The same sums as the kernels above, in the same order, with what does not
 // depend on the species hoisted out of the species loop: the operator
-// coefficients, the molar factors w / W_gp and sum grad * W_gp, the correction
-// velocity at each face; per species only Y and D at the cell and its six
+// coefficients, the molar factors w / W_gp and sum grad * W_gp (the molar
+// term's w (Y / W_gp) becomes (w / W_gp) Y: no division per species), the
+// correction velocity at each face; per species only Y and D at the cell and its six
 // neighbours are loaded (each once), instead of once per face and stencil
 // point.
 
@@ -424,6 +425,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
     const int64_t step[3] = {1, sy, sz};
     // wall faces carry no flux (the z faces of a non-wall cell are never walls)
     const bool live[3][2] = {{true, !wall(g, p.mask, i + 1, j)}, {true, !wall(g, p.mask, i, j + 1)}, {true, true}};
+    // per face (d, f) what does not depend on the species: the interpolation
+    // and gradient weights of its two points, the correction velocity and,
+    // for the molar term, w_m / W_gp,m and sum_m grad_m W_gp,m (the
+    // reference's SC / W_gp then reads (w_m / W_gp,m) Y_m: no division in the
+    // species loop)
+    double fw0[3][2], fw1[3][2], fg0[3][2], fg1[3][2], fcv[3][2], fu0[3][2], fu1[3][2], fgw[3][2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+      for (int f = 0; f < 2; ++f) {
+        // the face cell is (i, j, k) + f along d; its two points are
+        // (-1, 0) for f = 0 and (0, +1) for f = 1
+        const int fi = d == 0 ? i + f : i, fj = d == 1 ? j + f : j;
+        const int64_t fo = o + f * step[d];
+        double w0, w1;
+        if (d == 0) {
+          w0 = op(isx, fi, fi0, fni, fj, g.jmino_, g.nyo, -1, -1);
+          w1 = op(isx, fi, fi0, fni, fj, g.jmino_, g.nyo, 0, -1);
+        } else {
+          const double* ws = d == 1 ? isy : isz;
+          w0 = op(ws, fi, g.imino_, g.nxo, fj, fj0, fnj, -1, -1);
+          w1 = op(ws, fi, g.imino_, g.nxo, fj, fj0, fnj, 0, -1);
+        }
+        const double* gs = d == 0 ? grx : (d == 1 ? gry : grz);
+        const double g0 = op(gs, fi, fi0, fni, fj, fj0, fnj, -1, -1);
+        const double g1 = op(gs, fi, fi0, fni, fj, fj0, fnj, 0, -1);
+        fw0[d][f] = w0;
+        fw1[d][f] = w1;
+        fg0[d][f] = g0;
+        fg1[d][f] = g1;
+        fcv[d][f] = -cvel[(int64_t)d * g.plane + fo];
+        if (p.molar) {
+          const double wa = wgp[fo - step[d]], wb = wgp[fo];
+          fu0[d][f] = w0 / wa;
+          fu1[d][f] = w1 / wb;
+          fgw[d][f] = g0 * wa + g1 * wb;
+        } else {
+          fu0[d][f] = fu1[d][f] = fgw[d][f] = 0.0;
+        }
+      }
+    }
     double eT = 0.0;  // temperature term before the cp-gradient part and / cp
 #pragma unroll 1
     for (int sp = 0; sp < N; ++sp) {
@@ -437,24 +479,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
         const double ym = Y[o - step[d]], yp = Y[o + step[d]], dm = D[o - step[d]], dp = D[o + step[d]];
 #pragma unroll
         for (int f = 0; f < 2; ++f) {
-          // the face cell is (i, j, k) + f along d; its two points are
-          // (-1, 0) for f = 0 and (0, +1) for f = 1
-          const int fi = d == 0 ? i + f : i, fj = d == 1 ? j + f : j;
-          const int64_t fo = o + f * step[d];
           const double y0 = f ? yc : ym, y1 = f ? yp : yc;
           const double d0 = f ? dc : dm, d1 = f ? dp : dc;
-          double w0, w1, g0, g1;
-          if (d == 0) {
-            w0 = op(isx, fi, fi0, fni, fj, g.jmino_, g.nyo, -1, -1);
-            w1 = op(isx, fi, fi0, fni, fj, g.jmino_, g.nyo, 0, -1);
-          } else {
-            const double* ws = d == 1 ? isy : isz;
-            w0 = op(ws, fi, g.imino_, g.nxo, fj, fj0, fnj, -1, -1);
-            w1 = op(ws, fi, g.imino_, g.nxo, fj, fj0, fnj, 0, -1);
-          }
-          const double* gs = d == 0 ? grx : (d == 1 ? gry : grz);
-          g0 = op(gs, fi, fi0, fni, fj, fj0, fnj, -1, -1);
-          g1 = op(gs, fi, fi0, fni, fj, fj0, fnj, 0, -1);
+          const double w0 = fw0[d][f], w1 = fw1[d][f], g0 = fg0[d][f], g1 = fg1[d][f];
           double syf = 0, a = 0, b = 0;
           syf += w0 * y0;
           a += w0 * d0;
@@ -462,16 +489,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
           syf += w1 * y1;
           a += w1 * d1;
           b += g1 * y1;
-          double c = -cvel[(int64_t)d * g.plane + fo] * syf;
-          if (p.molar) {
-            const double wa = wgp[fo - step[d]], wb = wgp[fo];
-            double iyw = 0, gw = 0;
-            iyw += w0 * (y0 / wa);
-            gw += g0 * wa;
-            iyw += w1 * (y1 / wb);
-            gw += g1 * wb;
-            c += a * iyw * gw;
-          }
+          double c = fcv[d][f] * syf;
+          if (p.molar) c += a * (fu0[d][f] * y0 + fu1[d][f] * y1) * fgw[d][f];
           fc[d][f] = live[d][f] ? c : 0.0;
           ff[d][f] = live[d][f] ? a * b : 0.0;
         }
