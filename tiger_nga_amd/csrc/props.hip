@@ -26,8 +26,6 @@
 #include "chem_device.h"
 #include "chem_kernels.h"
 
-// constant address space: uniform loads from it are scalar loads
-#define TGR_CONST __attribute__((address_space(4)))
 
 namespace tgr {
 

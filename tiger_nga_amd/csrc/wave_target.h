@@ -7,6 +7,7 @@
 #define TGR_OPAQUE_PTR "+r"
 #define TGR_GLOBAL
 #define TGR_LDS
+#define TGR_CONST
 #else
 #include <hip/hip_runtime.h>
 #define TGR_OPAQUE_PTR "+s"
@@ -15,4 +16,7 @@
 #define TGR_GLOBAL __attribute__((address_space(1)))
 // pointee qualifier of LDS: ds_read / ds_write instead of flat accesses
 #define TGR_LDS __attribute__((address_space(3)))
+// pointee qualifier of read-only device memory (the constant address space):
+// loads through it with a wave-uniform address are scalar loads
+#define TGR_CONST __attribute__((address_space(4)))
 #endif
