@@ -120,3 +120,53 @@ def test_properties_kernel_vs_oracle(wl, mapping, oracle):
     print("%s properties (mapping %d): max rel err %.3e over %d cells (ghosts included)"
           % (wl, mapping, worst, SC_[..., 0].size))
     assert worst < 1e-13
+
+
+@pytest.mark.gpu
+def test_properties_mappings_agree_on_full_block(oracle):
+    """The 128^3-cell GRI-3.0 block of the benchmark (uneven stiffness
+    classes): the thread-per-cell and wave-per-cell mappings agree cell by
+    cell to rounding on every output, and a sample of cells matches the
+    oracle."""
+    import torch
+    from tiger_nga_amd.finitechem import FiniteChem, Grid
+    nx = 128
+    mech, Y, T, rho, P, dt = inputs.states("gri30_uneven", seed=5)
+    nsp = Y.shape[1]
+    RHO_, SC_, isc_sc, isc_T = inputs.fortran_fields(Y, T, rho, nx, nx, nx)
+    del Y, T, rho, RHO_
+    g = Grid.block(nx, nx, nx, SC_.shape[3], isc_sc, isc_T)
+
+    def fe(shape):
+        st = [1]
+        for s in shape[:-1]:
+            st.append(st[-1] * s)
+        return torch.empty_strided(shape, tuple(st), dtype=torch.float64, device="cuda")
+    S_d = fe(SC_.shape)
+    S_d.copy_(torch.from_numpy(SC_))
+    Le = torch.linspace(0.7, 1.6, nsp, dtype=torch.float64, device="cuda")
+    fc = FiniteChem(mech)
+    out = {}
+    for mapping in (0, 1):
+        R_d, V_d, D_d = fe(SC_.shape[:3]), fe(SC_.shape[:3]), fe(SC_.shape)
+        fc.set_option("props_mapping", mapping)
+        fc.properties(S_d, g, P, Lewis=Le, RHO_=R_d, VISC_=V_d, DIFF_=D_d)
+        torch.cuda.synchronize()
+        out[mapping] = (R_d, V_d, D_d)
+    for a, b, name in zip(out[0], out[1], ("RHO", "VISC", "DIFF")):
+        assert bool(torch.isfinite(b).all()) and float(b.min()) > 0.0, name
+        rel = ((a - b).abs() / b.abs()).max().item()
+        print("\n%s: max relative difference between the mappings %.2e" % (name, rel))
+        assert rel <= 1e-13, (name, rel)
+    R, V, D = (t.cpu().numpy() for t in out[1])
+    rng = np.random.default_rng(2)
+    Lh = Le.cpu().numpy()
+    worst = 0.0
+    for c in rng.choice(nx ** 3, 200, replace=False):
+        i, j, k = c % nx, (c // nx) % nx, c // (nx * nx)
+        y = SC_[i, j, k, isc_sc - 1:isc_sc - 1 + nsp]
+        v, dsp, dT, r = oracle.properties(mech, y, SC_[i, j, k, isc_T - 1], P, Lewis=Lh)
+        worst = max(worst, abs(V[i, j, k] / v - 1), abs(R[i, j, k] / r - 1), abs(D[i, j, k, isc_T - 1] / dT - 1),
+                    np.abs(D[i, j, k, isc_sc - 1:isc_sc - 1 + nsp] / dsp - 1).max())
+    print("sample of 200 cells against the oracle: max rel err %.2e" % worst)
+    assert worst < 1e-13
