@@ -39,6 +39,15 @@ __device__ __forceinline__ double omegamu(double t) {
   return num / den;
 }
 
+// omegamu(t) as its numerator and denominator (both positive for t > 0), so
+// that a caller dividing by omegamu multiplies by the denominator instead
+__device__ __forceinline__ void omegamu_nd(double t, double& num, double& den) {
+  const double m1 = 3.3530622607, m2 = 2.53272006, m3 = 2.9024238575, m4 = 0.11186138893, m5 = 0.8662326188,
+               m6 = 1.3913958626, m7 = 3.158490576, m8 = 0.18973411754, m9 = 0.00018682962894;
+  num = m1 + t * (m2 + t * (m3 + t * m4));
+  den = m5 + t * (m6 + t * (m7 + t * (m8 + t * m9)));
+}
+
 template <int NEQ>
 __global__ __launch_bounds__(256) void props_kernel(PropsParams p) {
   constexpr int nsp = NEQ - 1;
@@ -102,7 +111,8 @@ __global__ __launch_bounds__(256) void props_kernel(PropsParams p) {
 // field access is coalesced).  ise / yw: 1/sqrt(eta_j) and Y_j W_mix / W_j of
 // all species in registers; the loop over k keeps one species' eta, cp and
 // pair sums at a time.  Sums run in species order (the wave kernel's are
-// tree reductions: rounding-level differences).
+// tree reductions) and eta = mucoeff sqrt(T) den / num takes omegamu's
+// numerator and denominator with one division: rounding-level differences.
 template <int NEQ>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void props_lane_kernel(PropsParams p) {
   constexpr int nsp = NEQ - 1;
@@ -142,7 +152,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void p
   if (p.visc) {
 #pragma unroll
     for (int q = 0; q < nsp; ++q) {
-      ise[q] = 1.0 / sqrt(pt[q] * st / omegamu(temp * pt[64 + q]));
+      double num, den;
+      omegamu_nd(temp * pt[64 + q], num, den);
+      ise[q] = sqrt(num / (pt[q] * st * den));  // 1 / sqrt(eta_j)
       yw[q] *= Wmix;
     }
   }
@@ -154,7 +166,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void p
 #pragma unroll 1
   for (int k = 0; k < nsp; ++k) {
     const double Y = ys[k * p.plane];
-    const double eta = pt[k] * st / omegamu(temp * pt[64 + k]);
+    double num, den;
+    omegamu_nd(temp * pt[64 + k], num, den);
+    const double eta = pt[k] * st * den / num;
     const double ywk = Y * iw[k] * Wmix;
     if (p.visc) {
       const double se = sqrt(eta);
